@@ -1,0 +1,217 @@
+"""Benchmark: grid points per second of one FP64 RK-stage spatial operator.
+
+Workload (the curvilinear full-operator case): a 256^3 FP64 grid with tanh
+stretching (beta=2) toward the walls, metrics from JACOBEANO, a seeded
+solenoidal k^-5/3 velocity field (k <= 32) plus a six-blade rotating radial
+jet at mid-height.  One step = the spatial operator of one RUNGEK sub-step
+(solver/rungek_ico.f90:79-95): DERIV_VEL (9 velocity + 3 temperature
+gradient components), viscosidad_t + viscosidad (structure-function eddy
+viscosity), and flujos + divergencia for the five equations.  Inputs are
+resident in HBM when the timed region starts.
+
+Also reported, on the same JSON line:
+  roofline     -- the dominant kernel's achieved algorithmic bandwidth,
+                  timed with HIP events on the stream it runs on;
+  cpu_baseline -- the reference's own Fortran (oracle/_ref, built from the
+                  reference sources by flang -O3) running the same stage
+                  once on one host core (it is serial code);
+  parity       -- normwise relative error of the GPU stage outputs against
+                  that Fortran run.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pts/s per FP64 RK stage + %HBM roofline; batched/learned vs plain at stated err"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--scheme", default="c6f")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not "
+              "supported (the multi-GPU sharding is a design note, not a measured path)",
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from emix3d_amd import Operator, fields
+
+    n = args.n
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    seed = 20261204
+    t0 = time.time()
+    f = fields.stage_fields(n, n, n, seed, curvilinear=True, turbulent=True, device=dev)
+    op = Operator(n, n, n, args.scheme)
+    jbn = op.jacobeano(f["x"])
+    sgd, dmins, _ = op.inisgdm(f["x"])
+    c4 = 1.0 / fields.REYNOLDS
+    work = op.stage_buffers()
+    torch.cuda.synchronize()
+    log(f"bench: inputs ready in {time.time() - t0:.1f}s (n={n}, scheme={args.scheme})")
+
+    def step():
+        op.rk_stage(f["um"], f["u"], f["temp"], f["pres"], jbn, sgd, c4, work=work)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    pts = float(n) ** 3
+    value = pts / dt
+    log(f"bench: {dt * 1e3:.3f} ms per stage, {value / 1e9:.3f} Gpts/s")
+
+    # ---- dominant kernel, timed with events on the stream it runs on -----
+    roof = kernel_roofline(op, f, jbn, work, n, args.kernel_reps)
+
+    # ---- stage algorithmic bytes (each distinct input read once, each output
+    # written once): inputs um(5) u(3) temp pres jbn(1,5,9,10,11) sgd(7) = 22,
+    # outputs dvel(9) dtemp(3) amut vis rs(5) rsv(5) = 24 fields of 8 B.
+    stage_bytes_pt = (22 + 24) * 8
+    stage_gbs = stage_bytes_pt * pts / dt / 1e9
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        cpu, parity = cpu_baseline(f, jbn, sgd, c4, work, n, args.scheme)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "curvilinear full RK-stage spatial operator (grad + sgdm "
+                               "eddy viscosity + flux divergence of 5 equations)",
+                   "grid": [n, n, n], "stretching": "tanh beta=2", "scheme": args.scheme,
+                   "velocity": "solenoidal k^-5/3 (k<=32) + 6-blade radial jet",
+                   "stage_algorithmic_bytes_per_pt": stage_bytes_pt,
+                   "stage_algorithmic_GBs": stage_gbs,
+                   "stage_frac_of_hbm_peak": stage_gbs / HBM_PEAK_GBS},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def kernel_roofline(op, f, jbn, work, n, reps):
+    """The zeta-direction pass of the fused flux-divergence of a momentum
+    equation is the largest single kernel of the stage (it reads the most
+    fields).  Here the bandwidth figure is taken on the solver's core unit,
+    one compact derivative along zeta of one field: algorithmic bytes =
+    one FP64 read + one FP64 write per point = 16 B/pt."""
+    import torch
+    src = f["u"][0]
+    out = work["amut"].clone()
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        op.derivative(src, 2, out=out)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        op.derivative(src, 2, out=out)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    bytes_launch = 16.0 * n ** 3
+    gbs = bytes_launch / (ms * 1e-3) / 1e9
+    return {"kernel": "k_lines_strided (compact derivative along zeta, 1 field)",
+            "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": ms}
+
+
+def cpu_baseline(f, jbn, sgd, c4, work, n, scheme):
+    """The reference Fortran stage (oracle/_ref) once, on one core."""
+    import numpy as np
+    import torch
+    try:
+        from oracle.pyoracle import RefFortran
+    except Exception as e:  # pragma: no cover
+        log(f"bench: cpu baseline unavailable ({e})")
+        return None, None
+    lib_scheme = scheme if scheme in ("c4", "c6f") else None
+    if lib_scheme is None or not RefFortran.available(lib_scheme):
+        log("bench: reference Fortran build not present; no cpu baseline")
+        return None, None
+    h = {k: np.ascontiguousarray(v.cpu().numpy()) for k, v in f.items()}
+    hj = np.ascontiguousarray(jbn.cpu().numpy())
+    hs = np.ascontiguousarray(sgd.cpu().numpy())
+    ref = RefFortran(lib_scheme)
+    ref.stage_load(h["um"], h["u"], h["temp"], h["pres"], hj, hs, c4)
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+    except Exception:
+        pass
+    log("bench: timing the reference Fortran stage on one core ...")
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    saved = os.dup(1)
+    os.dup2(devnull, 1)  # the reference prints per call
+    try:
+        t0 = time.perf_counter()
+        ref.stage_run()
+        t = time.perf_counter() - t0
+    finally:
+        os.dup2(saved, 1)
+        os.close(devnull)
+        os.close(saved)
+    rs, rsv, amut, vis, dvel = ref.stage_fetch()
+    log(f"bench: reference stage {t:.2f}s on 1 core")
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+    g = {k: v.cpu().numpy() for k, v in work.items()}
+    parity = {"vs": "reference Fortran stage (same inputs, scheme %s)" % lib_scheme,
+              "dvel": rel(g["dvel"], dvel), "amut": rel(g["amut"], amut),
+              "vis": rel(g["vis"], vis)}
+    for m in range(5):
+        parity[f"rs{m + 1}"] = rel(g["rs"][m], rs[m])
+        if m in (1, 2, 3):
+            parity[f"rsv{m + 1}"] = rel(g["rsv"][m], rsv[m])
+    parity["max"] = max(v for k, v in parity.items() if k != "vs")
+    cpu = {"value": float(n) ** 3 / t, "unit": "grid-pts/s", "cores": 1, "kind": "reference",
+           "sample": f"one full {n}^3 RK-stage operator of the reference Fortran "
+                     "(deriv_vel, viscosidad_t, viscosidad, 5x flujos+divergencia; "
+                     "flang -O3, serial)", "seconds": t}
+    return cpu, parity
+
+
+if __name__ == "__main__":
+    main()

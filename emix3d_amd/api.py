@@ -1,0 +1,206 @@
+"""Python host interface over libemix3d_hip.so.
+
+Method names mirror the reference's Fortran subroutines on the hot path
+(solver/deriv_vel.f90 DERIV_VEL, solver/flujos_compact.f90 flujos,
+solver/divergencia.f90 divergencia, solver/sgdm_incomp.f90 viscosidad_t,
+solver/jacob_malla.f90 JACOBEANO, solver/advectLS.f90 advectLevelSet) and
+keep their argument meaning.  Fields are torch float64 tensors on the GPU,
+C-contiguous with shape (nz, ny, nx) or (ncomp, nz, ny, nx): the same memory
+as the reference's column-major (nx, ny, nz[, ncomp]) arrays.  PyTorch only
+provides device memory and the stream; every operator runs in the HIP
+kernels of the library.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+SCHEMES = {"c4": 0, "c6f": 1, "c6c": 2}
+KIND_WAVE, KIND_STRIDED = 0, 1
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("expected a torch tensor")
+    if t.dtype != torch.float64:
+        raise TypeError(f"expected float64, got {t.dtype}")
+    if not t.is_cuda:
+        raise ValueError("expected a GPU tensor (the operator has no CPU path)")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Operator:
+    """One grid (nx, ny, nz) and compact scheme: an emx_ctx."""
+
+    def __init__(self, nx, ny, nz, scheme="c6c", device=None):
+        if isinstance(scheme, str):
+            scheme = SCHEMES[scheme.lower()]
+        self.shape = (nx, ny, nz)
+        self.scheme = scheme
+        self.device = torch.device("cuda", torch.cuda.current_device() if device is None
+                                   else torch.device(device).index or 0)
+        self._ctx = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.call("emx_create", ctypes.byref(self._ctx), nx, ny, nz, scheme,
+                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def close(self):
+        if self._ctx:
+            _lib.lib().emx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ utils
+    def _stream(self):
+        _lib.call("emx_set_stream", self._ctx,
+                  ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+
+    def _empty(self, ncomp=None):
+        nx, ny, nz = self.shape
+        shape = (nz, ny, nx) if ncomp is None else (ncomp, nz, ny, nx)
+        return torch.empty(shape, dtype=torch.float64, device=self.device)
+
+    def _check(self, t, ncomp=None, name="field"):
+        nx, ny, nz = self.shape
+        want = (nz, ny, nx) if ncomp is None else (ncomp, nz, ny, nx)
+        if tuple(t.shape) != want:
+            raise ValueError(f"{name}: shape {tuple(t.shape)} != {want}")
+        return _ptr(t)
+
+    def default_ih(self):
+        nx, ny, nz = self.shape
+        return (float(nx - 1), float(ny - 1), float(nz - 1))
+
+    def set_plan(self, direction, kind, chunk):
+        _lib.call("emx_set_plan", self._ctx, direction, kind, chunk)
+
+    def get_plan(self, direction):
+        k, c = ctypes.c_int(), ctypes.c_int()
+        _lib.call("emx_get_plan", self._ctx, direction, ctypes.byref(k), ctypes.byref(c))
+        return k.value, c.value
+
+    # -------------------------------------------------------------- operators
+    def derivative(self, f, direction, inv_h=None, out=None):
+        """Compact first derivative along direction 0/1/2 (derivnper per line)."""
+        ih = self.default_ih()[direction] if inv_h is None else inv_h
+        out = self._empty() if out is None else out
+        self._stream()
+        _lib.call("emx_deriv", self._ctx, direction, self._check(f), self._check(out), ih)
+        return out
+
+    def deriv_vel(self, u, temp=None, ih=None, dvel=None, dtemp=None):
+        """DERIV_VEL: dvel (9 comps) and dtemp (3 comps) in computational space."""
+        ih = self.default_ih() if ih is None else ih
+        dvel = self._empty(9) if dvel is None else dvel
+        if temp is not None and dtemp is None:
+            dtemp = self._empty(3)
+        self._stream()
+        _lib.call("emx_grad", self._ctx, self._check(u, 3, "u"),
+                  None if temp is None else self._check(temp),
+                  self._check(dvel, 9, "dvel"),
+                  None if dtemp is None else self._check(dtemp, 3, "dtemp"), *ih)
+        return dvel, dtemp
+
+    def jacobeano(self, x, jbn=None):
+        """JACOBEANO: the 11 metric components from the coordinates x (3 comps)."""
+        jbn = self._empty(11) if jbn is None else jbn
+        self._stream()
+        _lib.call("emx_metrics", self._ctx, self._check(x, 3, "x"), self._check(jbn, 11, "jbn"))
+        return jbn
+
+    def inisgdm(self, x):
+        """INISGDM: the 7 SGS geometry fields, dmins and deltamax."""
+        sgd = self._empty(7)
+        dmins = self._empty()
+        dmax = ctypes.c_double()
+        self._stream()
+        _lib.call("emx_sgs_geometry", self._ctx, self._check(x, 3, "x"), _ptr(sgd),
+                  _ptr(dmins), ctypes.byref(dmax))
+        return sgd, dmins, dmax.value
+
+    def viscosidad_t(self, u, dvel, jbn, sgd, c4, amut=None, vis=None, vis5=None,
+                     want_vis5=True):
+        """viscosidad_t + viscosidad: amut, vis (= c4 + amut) and vis5."""
+        amut = self._empty() if amut is None else amut
+        vis = self._empty() if vis is None else vis
+        if want_vis5 and vis5 is None:
+            vis5 = self._empty()
+        self._stream()
+        _lib.call("emx_sgdm", self._ctx, self._check(u, 3, "u"), self._check(dvel, 9, "dvel"),
+                  self._check(jbn, 11, "jbn"), self._check(sgd, 7, "sgd"), float(c4),
+                  _ptr(amut), _ptr(vis), _ptr(vis5))
+        return amut, vis, vis5
+
+    def flujos(self, neq, um, u, pres, vis, jbn, dvel, flux=None):
+        """flujos(neq): the 6 flux fields e, f, g, ev, fv, gv."""
+        flux = self._empty(6) if flux is None else flux
+        self._stream()
+        _lib.call("emx_flux", self._ctx, int(neq), self._check(um, 5, "um"),
+                  self._check(u, 3, "u"), self._check(pres), self._check(vis),
+                  self._check(jbn, 11, "jbn"), self._check(dvel, 9, "dvel"),
+                  self._check(flux, 6, "flux"))
+        return flux
+
+    def divergencia(self, flux, jbn, ih=None, rs=None, rsv=None):
+        """divergencia: rs and rsv from the 6 flux fields (not modified)."""
+        ih = self.default_ih() if ih is None else ih
+        rs = self._empty() if rs is None else rs
+        rsv = self._empty() if rsv is None else rsv
+        self._stream()
+        _lib.call("emx_divergence", self._ctx, self._check(flux, 6, "flux"),
+                  self._check(jbn, 11, "jbn"), *ih, _ptr(rs), _ptr(rsv))
+        return rs, rsv
+
+    def flujos_divergencia(self, neq, um, u, pres, vis, jbn, dvel, ih=None, rs=None,
+                           rsv=None):
+        """flujos(neq) then divergencia(neq), fused (no flux fields stored)."""
+        ih = self.default_ih() if ih is None else ih
+        rs = self._empty() if rs is None else rs
+        rsv = self._empty() if rsv is None else rsv
+        self._stream()
+        _lib.call("emx_flux_div", self._ctx, int(neq), self._check(um, 5, "um"),
+                  self._check(u, 3, "u"), self._check(pres), _ptr(vis),
+                  self._check(jbn, 11, "jbn"), _ptr(dvel), *ih, _ptr(rs), _ptr(rsv))
+        return rs, rsv
+
+    def advect_level_set(self, pres, jbn, sobject, dxyz, deltamin):
+        """advectLevelSet: extrapolate pres into the solid (in place)."""
+        self._stream()
+        _lib.call("emx_ls_extrapolate", self._ctx, self._check(pres), self._check(jbn, 11, "jbn"),
+                  self._check(sobject), *map(float, dxyz), float(deltamin))
+        return pres
+
+    def advect_weno5(self, phi, vel, jbn, dxyz, dt):
+        """One SSP-RK3 WENO5 advection step of phi (in place)."""
+        self._stream()
+        _lib.call("emx_ls_advect_weno5", self._ctx, self._check(phi), self._check(vel, 3, "vel"),
+                  self._check(jbn, 11, "jbn"), *map(float, dxyz), float(dt))
+        return phi
+
+    def rk_stage(self, um, u, temp, pres, jbn, sgd, c4, ih=None, work=None):
+        """The spatial operator of one RUNGEK sub-step
+        (solver/rungek_ico.f90:79-95): DERIV_VEL, viscosidad_t, viscosidad and
+        flujos+divergencia for the five equations.  Returns a dict of device
+        tensors (rs, rsv: 5 comps each; dvel, dtemp, amut, vis)."""
+        w = self.stage_buffers() if work is None else work
+        self.deriv_vel(u, temp, ih, dvel=w["dvel"], dtemp=w["dtemp"])
+        self.viscosidad_t(u, w["dvel"], jbn, sgd, c4, amut=w["amut"], vis=w["vis"],
+                          want_vis5=False)
+        for m in range(1, 6):
+            self.flujos_divergencia(m, um, u, pres, w["vis"], jbn, w["dvel"], ih,
+                                    rs=w["rs"][m - 1], rsv=w["rsv"][m - 1])
+        return w
+
+    def stage_buffers(self):
+        return {"dvel": self._empty(9), "dtemp": self._empty(3), "amut": self._empty(),
+                "vis": self._empty(), "rs": self._empty(5), "rsv": self._empty(5)}

@@ -1,0 +1,516 @@
+// extern "C" boundary of libemix3d_hip.so (declared in include/emix3d.h).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "compact.hpp"
+#include "ops.hpp"
+
+using namespace emx;
+
+namespace {
+
+int check_ctx(const emx_ctx* ctx) {
+  if (!ctx) {
+    set_error("null context");
+    return EMX_ERR_ARG;
+  }
+  return EMX_OK;
+}
+
+#define CHECK_PTR(p)                            \
+  do {                                          \
+    if (!(p)) {                                 \
+      set_error("%s: null pointer " #p, __func__); \
+      return EMX_ERR_ARG;                       \
+    }                                           \
+  } while (0)
+
+int make_plan(emx_ctx* ctx, int dir, int kind, int C) {
+  const int n = dir == 0 ? ctx->d.nx : (dir == 1 ? ctx->d.ny : ctx->d.nz);
+  if (!valid_chunk(kind, C)) {
+    set_error("invalid line plan kind %d C %d", kind, C);
+    return EMX_ERR_ARG;
+  }
+  LineTab t{};
+  EMX_TRY(get_line_tab(ctx->scheme, n, C, kind, &t));
+  ctx->plan[dir].tab = t;
+  ctx->plan[dir].kind = kind;
+  ctx->plan[dir].C = C;
+  return EMX_OK;
+}
+
+void free_ctx(emx_ctx* c) {
+  if (!c) return;
+  (void)hipFree(c->acc);
+  (void)hipFree(c->sgs_tmp);
+  (void)hipFree(c->ls_buf);
+  (void)hipFree(c->stage);
+  delete c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* emx_last_error(void) { return emx::last_error(); }
+
+int emx_version(void) { return EMX_VERSION; }
+
+int emx_create(emx_ctx** out, int nx, int ny, int nz, int scheme,
+               void* stream) {
+  if (!out) {
+    set_error("emx_create: null output pointer");
+    return EMX_ERR_ARG;
+  }
+  *out = nullptr;
+  if (scheme < EMX_SCHEME_C4 || scheme > EMX_SCHEME_C6C) {
+    set_error("emx_create: unknown scheme %d", scheme);
+    return EMX_ERR_ARG;
+  }
+  const int nmin = scheme_min_n(scheme);
+  if (nx < nmin || ny < nmin || nz < nmin) {
+    set_error("emx_create: grid %dx%dx%d below the %d points per line the "
+              "scheme's boundary stencils need", nx, ny, nz, nmin);
+    return EMX_ERR_SIZE;
+  }
+  if (nx > 1024 || ny > 512 || nz > 512) {
+    set_error("emx_create: grid %dx%dx%d above the supported line lengths "
+              "(xi <= 1024, eta/zeta <= 512)", nx, ny, nz);
+    return EMX_ERR_SIZE;
+  }
+  emx_ctx* c = new emx_ctx();
+  c->scheme = scheme;
+  c->d = Dims{nx, ny, nz};
+  c->stream = (hipStream_t)stream;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    set_error("emx_create: no HIP device");
+    delete c;
+    return EMX_ERR_HIP;
+  }
+  for (int dir = 0; dir < 3; dir++) {
+    const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    int kind, C;
+    default_plan(dir, n, &kind, &C);
+    int rc = make_plan(c, dir, kind, C);
+    if (rc != EMX_OK) {
+      free_ctx(c);
+      return rc;
+    }
+  }
+  const size_t N = (size_t)c->d.N();
+  c->ls_bytes = ls_scratch_doubles(c->d) * sizeof(double);
+  if (hipMalloc(&c->acc, 2 * N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->sgs_tmp, 4 * N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->ls_buf, c->ls_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("emx_create: device allocation of scratch for %zu points failed", N);
+    free_ctx(c);
+    return EMX_ERR_NOMEM;
+  }
+  *out = c;
+  return EMX_OK;
+}
+
+int emx_destroy(emx_ctx* ctx) {
+  EMX_TRY(check_ctx(ctx));
+  free_ctx(ctx);
+  return EMX_OK;
+}
+
+int emx_set_stream(emx_ctx* ctx, void* stream) {
+  EMX_TRY(check_ctx(ctx));
+  ctx->stream = (hipStream_t)stream;
+  return EMX_OK;
+}
+
+int emx_set_plan(emx_ctx* ctx, int dir, int kind, int C) {
+  EMX_TRY(check_ctx(ctx));
+  if (dir < 0 || dir > 2) {
+    set_error("emx_set_plan: bad direction %d", dir);
+    return EMX_ERR_ARG;
+  }
+  return make_plan(ctx, dir, kind, C);
+}
+
+int emx_get_plan(const emx_ctx* ctx, int dir, int* kind, int* C) {
+  EMX_TRY(check_ctx(ctx));
+  if (dir < 0 || dir > 2 || !kind || !C) {
+    set_error("emx_get_plan: bad arguments");
+    return EMX_ERR_ARG;
+  }
+  *kind = ctx->plan[dir].kind;
+  *C = ctx->plan[dir].C;
+  return EMX_OK;
+}
+
+int emx_sync(emx_ctx* ctx) {
+  EMX_TRY(check_ctx(ctx));
+  EMX_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  return EMX_OK;
+}
+
+int emx_deriv(emx_ctx* ctx, int dir, const double* f, double* df,
+              double inv_h) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(f);
+  CHECK_PTR(df);
+  if (dir < 0 || dir > 2) {
+    set_error("emx_deriv: bad direction %d", dir);
+    return EMX_ERR_ARG;
+  }
+  if (f == df) {
+    set_error("emx_deriv: output aliases input");
+    return EMX_ERR_ARG;
+  }
+  return op_deriv(ctx, dir, f, df, inv_h);
+}
+
+int emx_grad(emx_ctx* ctx, const double* u, const double* temp, double* dvel,
+             double* dtemp, double ihx, double ihy, double ihz) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(u);
+  CHECK_PTR(dvel);
+  if ((temp == nullptr) != (dtemp == nullptr)) {
+    set_error("emx_grad: temp and dtemp must both be given or both be NULL");
+    return EMX_ERR_ARG;
+  }
+  const double ih[3] = {ihx, ihy, ihz};
+  return op_grad(ctx, u, temp, dvel, dtemp, ih);
+}
+
+int emx_metrics(emx_ctx* ctx, const double* x, double* jbn) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(x);
+  CHECK_PTR(jbn);
+  int* bad = reinterpret_cast<int*>(ctx->sgs_tmp);
+  EMX_TRY(op_metrics(ctx, x, jbn, bad));
+  int hbad = 0;
+  EMX_TRY_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost,
+                             ctx->stream));
+  EMX_TRY_HIP(hipStreamSynchronize(ctx->stream));
+  if (hbad) {
+    set_error("emx_metrics: zero Jacobian (|J| <= 1e-20) in the grid");
+    return EMX_ERR_METRIC;
+  }
+  return EMX_OK;
+}
+
+int emx_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
+                     double* dmins, double* deltamax) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(x);
+  CHECK_PTR(sgd);
+  if (ctx->d.nx < 2 || ctx->d.ny < 2 || ctx->d.nz < 2) {
+    set_error("emx_sgs_geometry: need at least 2 points per direction");
+    return EMX_ERR_SIZE;
+  }
+  const int nb = 1024;
+  double* bm = ctx->sgs_tmp;  // nb partial maxima
+  EMX_TRY(op_sgs_geometry(ctx, x, sgd, dmins, bm, nb));
+  if (deltamax) {
+    double h[nb];
+    EMX_TRY_HIP(hipMemcpyAsync(h, bm, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    EMX_TRY_HIP(hipStreamSynchronize(ctx->stream));
+    double m = 0.0;
+    for (int b = 0; b < nb; b++) m = h[b] > m ? h[b] : m;
+    *deltamax = m;
+  }
+  return EMX_OK;
+}
+
+int emx_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
+             const double* jbn, const double* sgd, double c4, double* amut,
+             double* vis, double* vis5) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(u);
+  CHECK_PTR(dvel);
+  CHECK_PTR(jbn);
+  CHECK_PTR(sgd);
+  CHECK_PTR(amut);
+  CHECK_PTR(vis);
+  if (ctx->d.nx < 2 || ctx->d.ny < 2 || ctx->d.nz < 2) {
+    set_error("emx_sgdm: need at least 2 points per direction");
+    return EMX_ERR_SIZE;
+  }
+  return op_sgdm(ctx, u, dvel, jbn, sgd, c4, amut, vis, vis5);
+}
+
+int emx_flux(emx_ctx* ctx, int neq, const double* um, const double* u,
+             const double* pres, const double* vis, const double* jbn,
+             const double* dvel, double* flux) {
+  EMX_TRY(check_ctx(ctx));
+  if (neq < 1 || neq > 5) {
+    set_error("emx_flux: equation index %d outside 1..5", neq);
+    return EMX_ERR_ARG;
+  }
+  CHECK_PTR(um);
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  CHECK_PTR(vis);
+  CHECK_PTR(jbn);
+  CHECK_PTR(dvel);
+  CHECK_PTR(flux);
+  return op_flux(ctx, neq, FluxPtrs{um, u, pres, vis, jbn, dvel}, flux);
+}
+
+int emx_divergence(emx_ctx* ctx, const double* flux, const double* jbn,
+                   double ihx, double ihy, double ihz, double* rs,
+                   double* rsv) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(flux);
+  CHECK_PTR(jbn);
+  CHECK_PTR(rs);
+  CHECK_PTR(rsv);
+  const double ih[3] = {ihx, ihy, ihz};
+  return op_divergence(ctx, flux, jbn, ih, rs, rsv);
+}
+
+int emx_flux_div(emx_ctx* ctx, int neq, const double* um, const double* u,
+                 const double* pres, const double* vis, const double* jbn,
+                 const double* dvel, double ihx, double ihy, double ihz,
+                 double* rs, double* rsv) {
+  EMX_TRY(check_ctx(ctx));
+  if (neq < 1 || neq > 5) {
+    set_error("emx_flux_div: equation index %d outside 1..5", neq);
+    return EMX_ERR_ARG;
+  }
+  CHECK_PTR(um);
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  CHECK_PTR(jbn);
+  CHECK_PTR(rs);
+  CHECK_PTR(rsv);
+  if (neq >= 2 && neq <= 4) {
+    CHECK_PTR(vis);
+    CHECK_PTR(dvel);
+  }
+  const double ih[3] = {ihx, ihy, ihz};
+  return op_flux_div(ctx, neq, FluxPtrs{um, u, pres, vis, jbn, dvel}, ih, rs, rsv);
+}
+
+int emx_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
+                       const double* sobject, double dx, double dy,
+                       double dz, double deltamin) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(pres);
+  CHECK_PTR(jbn);
+  CHECK_PTR(sobject);
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0)) {
+    set_error("emx_ls_extrapolate: spacings must be positive");
+    return EMX_ERR_ARG;
+  }
+  return op_ls_extrapolate(ctx, pres, jbn, sobject, dx, dy, dz, deltamin);
+}
+
+int emx_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
+                        const double* jbn, double dx, double dy, double dz,
+                        double dt) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(phi);
+  CHECK_PTR(vel);
+  CHECK_PTR(jbn);
+  if (!(dx > 0.0) || !(dy > 0.0) || !(dz > 0.0)) {
+    set_error("emx_ls_advect_weno5: spacings must be positive");
+    return EMX_ERR_ARG;
+  }
+  return op_ls_advect_weno5(ctx, phi, vel, jbn, dx, dy, dz, dt);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Host-pointer drop-ins.  A context per grid size (scheme C6C, the scheme of
+// the reference's CUDA path) is kept with a staging area, so repeated calls
+// allocate nothing; copies are synchronous like the reference's cudaMemcpy.
+
+namespace {
+
+std::mutex g_legacy_mu;
+
+emx_ctx* legacy_ctx(int scheme, int nx, int ny, int nz, size_t stage_doubles) {
+  static std::map<std::tuple<int, int, int, int, int>, emx_ctx*> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  auto key = std::make_tuple(dev, scheme, nx, ny, nz);
+  emx_ctx* c = nullptr;
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    c = it->second;
+  } else {
+    if (emx_create(&c, nx, ny, nz, scheme, nullptr) != EMX_OK) return nullptr;
+    cache[key] = c;
+  }
+  const size_t need = stage_doubles * sizeof(double);
+  if (c->stage_bytes < need) {
+    (void)hipFree(c->stage);
+    c->stage = nullptr;
+    c->stage_bytes = 0;
+    if (hipMalloc(&c->stage, need) != hipSuccess) {
+      set_error("staging allocation of %zu bytes failed", need);
+      return nullptr;
+    }
+    c->stage_bytes = need;
+  }
+  return c;
+}
+
+[[noreturn]] void legacy_fail(const char* who) {
+  fprintf(stderr, "%s error: %s\n", who, emx_last_error());
+  exit(-1);
+}
+
+#define LEG(call, who)                 \
+  do {                                 \
+    if ((call) != EMX_OK) legacy_fail(who); \
+  } while (0)
+
+int h2d(emx_ctx* c, double* dst, const double* src, size_t n) {
+  EMX_TRY_HIP(hipMemcpyAsync(dst, src, n * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  return EMX_OK;
+}
+int d2h(emx_ctx* c, double* dst, const double* src, size_t n) {
+  EMX_TRY_HIP(hipMemcpyAsync(dst, src, n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  return EMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cudaDeriv(double* dVel, double* dTemp, double* vel, double* temp,
+               double* amut, double* dxyzsgd, double* jbn, double deltaX,
+               double deltaY, double deltaZ, int Nx, int Ny, int Nz) {
+  (void)amut;
+  (void)dxyzsgd;
+  (void)jbn;
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  const size_t N = (size_t)Nx * Ny * Nz;
+  emx_ctx* c = legacy_ctx(EMX_SCHEME_C6C, Nx, Ny, Nz, 16 * N);
+  if (!c) legacy_fail("cudaDeriv");
+  double* du = c->stage;       // 3N
+  double* dt = du + 3 * N;     // N
+  double* ddv = dt + N;        // 9N
+  double* ddt = ddv + 9 * N;   // 3N
+  LEG(h2d(c, du, vel, 3 * N), "cudaDeriv");
+  LEG(h2d(c, dt, temp, N), "cudaDeriv");
+  // the reference passes 1/deltax and its kernels multiply by 1.0/delta
+  // (src/Schemes/derivCompact.cu:105)
+  LEG(emx_grad(c, du, dt, ddv, ddt, 1.0 / deltaX, 1.0 / deltaY, 1.0 / deltaZ),
+      "cudaDeriv");
+  LEG(d2h(c, dVel, ddv, 9 * N), "cudaDeriv");
+  LEG(d2h(c, dTemp, ddt, 3 * N), "cudaDeriv");
+  LEG(emx_sync(c), "cudaDeriv");
+}
+
+void cuDivergence(double* rs, double* rsv, double* e, double* f, double* g,
+                  double* ev, double* fv, double* gv, double* jbn,
+                  double deltaX, double deltaY, double deltaZ, int Nx, int Ny,
+                  int Nz) {
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  const size_t N = (size_t)Nx * Ny * Nz;
+  emx_ctx* c = legacy_ctx(EMX_SCHEME_C6C, Nx, Ny, Nz, 19 * N);
+  if (!c) legacy_fail("cuDivergence");
+  double* flux = c->stage;    // 6N
+  double* dj = flux + 6 * N;  // 11N (comps 1,5,9,10,11 filled)
+  double* drs = dj + 11 * N;  // 2N
+  double* src[6] = {e, f, g, ev, fv, gv};
+  for (int q = 0; q < 6; q++) LEG(h2d(c, flux + q * N, src[q], N), "cuDivergence");
+  const int comps[5] = {0, 4, 8, 9, 10};  // jbn 1,5,9,10,11 (cuDivergence.cu:158-163)
+  for (int q = 0; q < 5; q++)
+    LEG(h2d(c, dj + comps[q] * N, jbn + comps[q] * N, N), "cuDivergence");
+  LEG(emx_divergence(c, flux, dj, 1.0 / deltaX, 1.0 / deltaY, 1.0 / deltaZ,
+                     drs, drs + N),
+      "cuDivergence");
+  LEG(d2h(c, rs, drs, N), "cuDivergence");
+  LEG(d2h(c, rsv, drs + N, N), "cuDivergence");
+  LEG(emx_sync(c), "cuDivergence");
+}
+
+void advect_CUDA(double* press, double* jbn, double* sobject, double deltaX,
+                 double deltaY, double deltaZ, int Nx, int Ny, int Nz,
+                 double deltamin) {
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  const size_t N = (size_t)Nx * Ny * Nz;
+  // the level set needs no derivative table; any scheme whose minimum line
+  // length the grid meets will do for the context
+  emx_ctx* c = legacy_ctx(EMX_SCHEME_C6C, Nx, Ny, Nz, 13 * N);
+  if (!c) legacy_fail("advect_CUDA");
+  double* dp = c->stage;
+  double* dj = dp + N;        // 11N (comps 1,5,9 filled)
+  double* ds = dj + 11 * N;
+  LEG(h2d(c, dp, press, N), "advect_CUDA");
+  const int comps[3] = {0, 4, 8};
+  for (int q = 0; q < 3; q++)
+    LEG(h2d(c, dj + comps[q] * N, jbn + comps[q] * N, N), "advect_CUDA");
+  LEG(h2d(c, ds, sobject, N), "advect_CUDA");
+  LEG(emx_ls_extrapolate(c, dp, dj, ds, deltaX, deltaY, deltaZ, deltamin),
+      "advect_CUDA");
+  LEG(d2h(c, press, dp, N), "advect_CUDA");
+  LEG(emx_sync(c), "advect_CUDA");
+}
+
+int emx_host_flujos(int neq, const double* um, const double* u,
+                    const double* pres, const double* vis, const double* jbn,
+                    const double* dvel, double* e, double* f, double* g,
+                    double* ev, double* fv, double* gv, int Nx, int Ny,
+                    int Nz) {
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  if (!um || !u || !pres || !vis || !jbn || !dvel || !e || !f || !g || !ev ||
+      !fv || !gv) {
+    set_error("emx_host_flujos: null pointer");
+    return EMX_ERR_ARG;
+  }
+  const size_t N = (size_t)Nx * Ny * Nz;
+  emx_ctx* c = legacy_ctx(EMX_SCHEME_C6C, Nx, Ny, Nz, 36 * N);
+  if (!c) return EMX_ERR_NOMEM;
+  double* dum = c->stage;       // 5N
+  double* du = dum + 5 * N;     // 3N
+  double* dp = du + 3 * N;      // N
+  double* dvs = dp + N;         // N
+  double* dj = dvs + N;         // 11N
+  double* ddv = dj + 11 * N;    // 9N
+  double* dfl = ddv + 9 * N;    // 6N
+  EMX_TRY(h2d(c, dum, um, 5 * N));
+  EMX_TRY(h2d(c, du, u, 3 * N));
+  EMX_TRY(h2d(c, dp, pres, N));
+  EMX_TRY(h2d(c, dvs, vis, N));
+  EMX_TRY(h2d(c, dj, jbn, 11 * N));
+  EMX_TRY(h2d(c, ddv, dvel, 9 * N));
+  EMX_TRY(emx_flux(c, neq, dum, du, dp, dvs, dj, ddv, dfl));
+  double* dst[6] = {e, f, g, ev, fv, gv};
+  for (int q = 0; q < 6; q++) EMX_TRY(d2h(c, dst[q], dfl + q * N, N));
+  return emx_sync(c);
+}
+
+int emx_host_sgdm(const double* u, const double* dvel, const double* jbn,
+                  const double* sgd, double c4, double* amut, double* vis,
+                  double* vis5, int Nx, int Ny, int Nz) {
+  std::lock_guard<std::mutex> lk(g_legacy_mu);
+  if (!u || !dvel || !jbn || !sgd || !amut || !vis) {
+    set_error("emx_host_sgdm: null pointer");
+    return EMX_ERR_ARG;
+  }
+  const size_t N = (size_t)Nx * Ny * Nz;
+  emx_ctx* c = legacy_ctx(EMX_SCHEME_C6C, Nx, Ny, Nz, 33 * N);
+  if (!c) return EMX_ERR_NOMEM;
+  double* du = c->stage;      // 3N
+  double* ddv = du + 3 * N;   // 9N
+  double* dj = ddv + 9 * N;   // 11N
+  double* dsg = dj + 11 * N;  // 7N
+  double* dout = dsg + 7 * N; // 3N
+  EMX_TRY(h2d(c, du, u, 3 * N));
+  EMX_TRY(h2d(c, ddv, dvel, 9 * N));
+  EMX_TRY(h2d(c, dj, jbn, 11 * N));
+  EMX_TRY(h2d(c, dsg, sgd, 7 * N));
+  EMX_TRY(emx_sgdm(c, du, ddv, dj, dsg, c4, dout, dout + N,
+                   vis5 ? dout + 2 * N : nullptr));
+  EMX_TRY(d2h(c, amut, dout, N));
+  EMX_TRY(d2h(c, vis, dout + N, N));
+  if (vis5) EMX_TRY(d2h(c, vis5, dout + 2 * N, N));
+  return emx_sync(c);
+}
+
+}  // extern "C"

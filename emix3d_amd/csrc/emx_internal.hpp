@@ -1,0 +1,107 @@
+// Internal declarations of libemix3d_hip: status handling, the per-context
+// state and the line-solver tables.  Everything here is FP64 and uses the
+// reference's Fortran column-major layout: element (i,j,k,c) of an
+// (nx,ny,nz,nc) array sits at i + nx*(j + ny*(k + nz*c))
+// (src/Engine/cuArray.cu:119-124, solver/cuDeriv.cu:588-590).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/emix3d.h"
+
+namespace emx {
+
+void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define EMX_TRY_HIP(call)                                                   \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::emx::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,   \
+                       hipGetErrorString(e_));                              \
+      return EMX_ERR_HIP;                                                   \
+    }                                                                       \
+  } while (0)
+
+#define EMX_TRY(call)                 \
+  do {                                \
+    int rc_ = (call);                 \
+    if (rc_ != EMX_OK) return rc_;    \
+  } while (0)
+
+// Kinds of line solver launch (see compact.hpp).
+enum LineKind { KIND_WAVE = 0, KIND_STRIDED = 1 };
+
+// Device tables of one (scheme, n, chunk) combination.  All arrays live in
+// one device allocation; pointers are device addresses.  With the Thomas
+// factors of the constant tridiagonal matrix (bet_l, gam_l) the solve of a
+// line chunked into P pieces of C points is:
+//   forward  y_l = r_l*ib_l - m_l*y_{l-1}, computed per chunk from a zero
+//            carry and corrected by y_l += H_l * Y_{c-1};
+//   backward x_l = y_l - gam_{l+1}*x_{l+1}, per chunk from a zero carry,
+//            corrected by x_l += G_l * X_{c+1};
+// the chunk carries Y/X come from the weights Wf/Wb (strided kind) or from
+// log2 wave-scan steps with multipliers Af/Ab (wave kind).
+struct LineTab {
+  int n;      // line length
+  int C;      // points per chunk
+  int P;      // chunks per line
+  int kind;   // LineKind
+  const double* ib;   // [n]    1/bet_l
+  const double* m;    // [n]    a_l/bet_l
+  const double* gam;  // [n+1]  gam_l (gam_0 = gam_n = 0)
+  const double* H;    // [n]    prod_{t=s_c..l} (-m_t)
+  const double* G;    // [n]    prod_{t=l+1..e_c} (-gam_t)
+  const double* Wf;   // [P*P]  strided kind: forward carry weights
+  const double* Wb;   // [P*P]  strided kind: backward carry weights
+  const double* Af;   // [6*64] wave kind: forward scan multipliers
+  const double* Ab;   // [6*64] wave kind: backward scan multipliers
+};
+
+// Builds (once per device) and returns the table.  Thread-safe.
+int get_line_tab(int scheme, int n, int C, int kind, LineTab* out);
+// Host copy of the Thomas factors (for tests and the adjoint).
+int host_thomas(int scheme, int n, std::vector<double>& a,
+                std::vector<double>& b, std::vector<double>& c);
+int scheme_min_n(int scheme);
+
+// Launch configuration of one direction of one context.
+struct DirPlan {
+  LineTab tab;
+  int kind;
+  int C;
+};
+
+struct Dims {
+  int nx, ny, nz;
+  __host__ __device__ long long N() const { return (long long)nx * ny * nz; }
+};
+
+}  // namespace emx
+
+// The opaque context of the C ABI.
+struct emx_ctx {
+  int device = 0;
+  int scheme = EMX_SCHEME_C6C;
+  emx::Dims d{0, 0, 0};
+  hipStream_t stream = nullptr;
+  emx::DirPlan plan[3];
+  // persistent scratch (device): two accumulators for the fused
+  // flux-divergence, four fields for the eddy-viscosity kernels.
+  double* acc = nullptr;       // 2*N
+  double* sgs_tmp = nullptr;   // 4*N
+  // level-set scratch (allocated lazily on first level-set call)
+  double* ls_buf = nullptr;
+  size_t ls_bytes = 0;
+  // staging for the host-pointer entry points
+  double* stage = nullptr;
+  size_t stage_bytes = 0;
+};

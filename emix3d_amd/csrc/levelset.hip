@@ -1,0 +1,369 @@
+// Level-set kernels: pressure extrapolation into the solid (the path the
+// reference's advectLevelSet really runs) and WENO5 / SSP-RK3 advection of
+// phi (cuAdvectLsJB).  Both work on the 5-cell mirror-padded domain that
+// advect_CUDA builds (src/LevelSet/lsTransportEc.cu:39-166), with the
+// reference's per-point boundary substitutions on the outermost padded
+// layers restated exactly.  Unlike the reference (floor-divided grids,
+// lsTransportEc.cu:58-79), every point is computed, for any size.
+#include <cmath>
+
+#include "ops.hpp"
+
+namespace emx {
+
+namespace {
+
+constexpr int GC = 5;  // gcells, lsTransportEc.cu:39
+
+struct PDims {
+  int x, y, z;
+  __host__ __device__ long long N() const { return (long long)x * y * z; }
+};
+
+__device__ __forceinline__ int mirror(int i, int n) {
+  // padded index -> interior index of the even mirror used by
+  // cuGhostCellsMirror3d{X,Y,Z} (lsTools.cu:482-585)
+  int s = i < GC ? 2 * GC - 1 - i : (i >= n + GC ? 2 * (n + GC) - 1 - i : i);
+  return s - GC;
+}
+
+// cuSwapToGhost + the three mirror passes, as one gather.
+__global__ void k_pad(const double* in, int nx, int ny, int nz, double* out,
+                      PDims g) {
+  const long long NG = g.N();
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < NG;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % g.x), j = (int)((p / g.x) % g.y),
+              k = (int)(p / ((long long)g.x * g.y));
+    const int is = mirror(i, nx), js = mirror(j, ny), ks = mirror(k, nz);
+    out[p] = in[(long long)is + (long long)nx * ((long long)js + (long long)ny * ks)];
+  }
+}
+
+// cuSwapFromGhost (lsTools.cu:798-822).
+__global__ void k_unpad(const double* in, PDims g, int nx, int ny, int nz,
+                        double* out) {
+  const long long N = (long long)nx * ny * nz;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % nx), j = (int)((p / nx) % ny),
+              k = (int)(p / ((long long)nx * ny));
+    out[p] = in[(long long)(i + GC) + (long long)g.x * ((long long)(j + GC) + (long long)g.y * (k + GC))];
+  }
+}
+
+// DevFirstOrder_LS (src/LevelSet/extrapol.cu:105-153).
+__global__ void k_ls_grad1(const double* S, const double* j1, const double* j5,
+                           const double* j9, PDims g, double iX, double iY,
+                           double iZ, double* dphi) {
+  const long long NG = g.N();
+  const long long sxy = (long long)g.x * g.y;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(id % g.x), y = (int)((id / g.x) % g.y), z = (int)(id / sxy);
+    long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x,
+              kp = id + sxy, km = id - sxy;
+    double factor = 0.5;
+    if (x == 0) { im = id; factor = 1.0; }
+    if (y == 0) { jm = id; factor = 1.0; }
+    if (z == 0) { km = id; factor = 1.0; }
+    if (x == g.x - 1) { ip = id; factor = 1.0; }
+    if (y == g.y - 1) { jp = id; factor = 1.0; }
+    if (z == g.z - 1) { kp = id; factor = 1.0; }
+    dphi[id] = factor * iX * j1[id] * (S[ip] - S[im]);
+    dphi[NG + id] = factor * iY * j5[id] * (S[jp] - S[jm]);
+    dphi[2 * NG + id] = factor * iZ * j9[id] * (S[kp] - S[km]);
+  }
+}
+
+// extrapolKernel (extrapol.cu:5-103, extFlag>0) fused with the SSP-RK3
+// stage update RunGK_{First,Second,Third}S (extrapol.cu:155-212):
+//   stage 0: out = ext - dt*rs
+//   stage 1: out = 0.75*P + 0.25*(ext - dt*rs)
+//   stage 2: out = (P + 2*(ext - dt*rs))/3      (out may alias P)
+// rs is exactly zero where phiS <= 0 (so = 0 and finite operands), so the
+// stencil is skipped there.
+__global__ void k_ls_extrap_stage(int stage, const double* ext,
+                                  const double* P, const double* S,
+                                  const double* j1, const double* j5,
+                                  const double* j9, const double* dphi,
+                                  PDims g, double iX, double iY, double iZ,
+                                  double dt, double* out) {
+  const long long NG = g.N();
+  const long long sxy = (long long)g.x * g.y;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
+       id += (long long)gridDim.x * blockDim.x) {
+    double rs = 0.0;
+    if (S[id] > 0.0) {
+      const int x = (int)(id % g.x), y = (int)((id / g.x) % g.y), z = (int)(id / sxy);
+      long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x,
+                kp = id + sxy, km = id - sxy;
+      long long id2 = id;
+      if (x == 0) { id2 = ip; im = id; }
+      if (y == 0) { id2 = jp; jm = id; }
+      if (z == 0) { id2 = kp; km = id; }
+      if (x == g.x - 1) { id2 = im; ip = id; }
+      if (y == g.y - 1) { id2 = jm; jp = id; }
+      if (z == g.z - 1) { id2 = km; kp = id; }
+      const double so = 1.0;
+      const double pdx = so * dphi[id], pdy = so * dphi[NG + id],
+                   pdz = so * dphi[2 * NG + id];
+      const double e2 = ext[id2];
+      const double xd = iX * j1[id] * (e2 - ext[im]);
+      const double yd = iY * j5[id] * (e2 - ext[jm]);
+      const double zd = iZ * j9[id] * (e2 - ext[km]);
+      const double xu = iX * j1[id] * (ext[ip] - e2);
+      const double yu = iY * j5[id] * (ext[jp] - e2);
+      const double zu = iZ * j9[id] * (ext[kp] - e2);
+      double s = (double)(pdx > 0.0) - (double)(pdx < 0.0);
+      double r = (0.5 * (s + 1.0) * xd + 0.5 * fabs(s - 1.0) * xu) * pdx;
+      s = (double)(pdy > 0.0) - (double)(pdy < 0.0);
+      r += (0.5 * (s + 1.0) * yd + 0.5 * fabs(s - 1.0) * yu) * pdy;
+      s = (double)(pdz > 0.0) - (double)(pdz < 0.0);
+      r += (0.5 * (s + 1.0) * zd + 0.5 * fabs(s - 1.0) * zu) * pdz;
+      rs = r;
+    }
+    const double ev = ext[id];
+    if (stage == 0) out[id] = ev - dt * rs;
+    else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (ev - dt * rs);
+    else out[id] = (P[id] + 2.0 * (ev - dt * rs)) / 3.0;
+  }
+}
+
+// Phi_x_WENO (src/LevelSet/lsTools.cu:9-50).
+__device__ __forceinline__ double weno5(double b1, double b2, double b3,
+                                        double b4, double b5) {
+  const double s1 = (13.0 / 12.0) * (b1 - 2.0 * b2 + b3) * (b1 - 2.0 * b2 + b3) +
+                    (0.25) * (b1 - 4.0 * b2 + 3.0 * b3) * (b1 - 4.0 * b2 + 3.0 * b3);
+  const double s2 = (13.0 / 12.0) * (b2 - 2.0 * b3 + b4) * (b2 - 2.0 * b3 + b4) +
+                    (0.25) * (b2 - b4) * (b2 - b4);
+  const double s3 = (13.0 / 12.0) * (b3 - 2.0 * b4 + b5) * (b3 - 2.0 * b4 + b5) +
+                    (0.25) * (3.0 * b3 - 4.0 * b4 + b5) * (3.0 * b3 - 4.0 * b4 + b5);
+  const double a1 = 0.1 / ((s1 + 1.0e-6) * (s1 + 1.0e-6));
+  const double a2 = 0.6 / ((s2 + 1.0e-6) * (s2 + 1.0e-6));
+  const double a3 = 0.3 / ((s3 + 1.0e-6) * (s3 + 1.0e-6));
+  const double o1 = a1 / (a1 + a2 + a3);
+  const double o2 = a2 / (a1 + a2 + a3);
+  const double o3 = a3 / (a1 + a2 + a3);
+  return ((o1 * (2.0 * b1 - 7.0 * b2 + 11.0 * b3) +
+           o2 * (-1.0 * b2 + 5.0 * b3 + 2.0 * b4) +
+           o3 * (2.0 * b3 + 5.0 * b4 - b5)) *
+          (1.0 / 6.0));
+}
+
+// Backward difference of phi along axis `ax` at point (x,y,z), with the
+// index-0 rule of Dev1thO_Downwind (lsTools.cu:53-95).
+__device__ __forceinline__ double bdiff(const double* phi, PDims g, int ax,
+                                        int x, int y, int z, double ih) {
+  const long long sxy = (long long)g.x * g.y;
+  const long long id = (long long)x + (long long)g.x * y + sxy * z;
+  if (ax == 0) {
+    if (x == 0) return ih * (phi[id + 1] - phi[id]);
+    return ih * (phi[id] - phi[id - 1]);
+  }
+  if (ax == 1) {
+    if (y == 0) return ih * (phi[id + g.x] - phi[id]);
+    return ih * (phi[id] - phi[id - g.x]);
+  }
+  if (z == 0) return ih * (phi[id + sxy] - phi[id]);
+  return ih * (phi[id] - phi[id - sxy]);
+}
+
+// One fused SSP-RK3 stage of cuAdvectLsJB (lsTransportEc.cu:243-399):
+// backward differences, the two JB WENO derivatives (PhiDevPlusParameterJB
+// lsTools.cu:587-673 -> "left", PhiDevMinusParameterJB :676-770 ->
+// "right"), the upwind right-hand side advect_RS_WENO (:361-417) and the
+// stage update.  src is the stage input, P the step's initial phi.
+__global__ void k_weno_stage(int stage, const double* src, const double* P,
+                             const double* vel, const double* j1,
+                             const double* j5, const double* j9, PDims g,
+                             double iX, double iY, double iZ, double dt,
+                             double* out) {
+  const long long NG = g.N();
+  const long long sxy = (long long)g.x * g.y;
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
+       id += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(id % g.x), y = (int)((id / g.x) % g.y), z = (int)(id / sxy);
+    // ---- left stencils (PhiDevPlusParameterJB), positions along each axis
+    int lx[5] = {x - 2, x - 1, x, x + 1, x + 2};
+    int ly[5] = {y - 2, y - 1, y, y + 1, y + 2};
+    int lz[5] = {z - 2, z - 1, z, z + 1, z + 2};
+    // y-coordinate that id_jm1 ends up at (used by the z quirk)
+    int jm1y = y - 1;
+    if (x == 0) { lx[1] = x; lx[0] = x + 1; }
+    if (x == 1) { lx[0] = lx[1]; }
+    if (x == g.x - 1) { lx[3] = x; lx[4] = lx[1]; }
+    if (x == g.x - 2) { lx[4] = lx[3]; }
+    if (y == 0) { ly[1] = y; ly[0] = y + 1; jm1y = y; }
+    if (y == 1) { ly[0] = ly[1]; }
+    if (y == g.y - 1) { ly[3] = y; ly[4] = ly[1]; }
+    if (y == g.y - 2) { ly[4] = ly[3]; }
+    bool zquirk = false;
+    if (z == 0) { lz[1] = z; lz[0] = z + 1; }
+    if (z == 1) { lz[0] = lz[1]; }
+    if (z == g.z - 1) { lz[3] = z; zquirk = true; }  // id_kp2 = id_jm1
+    if (z == g.z - 2) { lz[4] = lz[3]; }
+    double left[3], right[3];
+    {
+      const double m = j1[id];
+      double b[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 0, lx[q], y, z, iX) * m;
+      left[0] = weno5(b[0], b[1], b[2], b[3], b[4]);
+    }
+    {
+      const double m = j5[id];
+      double b[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 1, x, ly[q], z, iY) * m;
+      left[1] = weno5(b[0], b[1], b[2], b[3], b[4]);
+    }
+    {
+      const double m = j9[id];
+      double b[5];
+#pragma unroll
+      for (int q = 0; q < 4; q++) b[q] = bdiff(src, g, 2, x, y, lz[q], iZ) * m;
+      b[4] = zquirk ? bdiff(src, g, 2, x, jm1y, z, iZ) * m
+                    : bdiff(src, g, 2, x, y, lz[4], iZ) * m;
+      left[2] = weno5(b[0], b[1], b[2], b[3], b[4]);
+    }
+    // ---- right stencils (PhiDevMinusParameterJB): ip3, ip2, ip1, id, im1
+    int rx[5] = {x + 3, x + 2, x + 1, x, x - 1};
+    int ry[5] = {y + 3, y + 2, y + 1, y, y - 1};
+    int rz[5] = {z + 3, z + 2, z + 1, z, z - 1};
+    if (x == 0) { rx[4] = x; }
+    if (x == g.x - 1) { rx[2] = x; rx[1] = rx[4]; rx[0] = x - 2; }
+    if (x == g.x - 2) { rx[1] = rx[2]; rx[0] = x; }
+    if (x == g.x - 3) { rx[0] = rx[1]; }
+    if (y == 0) { ry[4] = y; }
+    if (y == g.y - 1) { ry[2] = y; ry[1] = ry[4]; ry[0] = y - 2; }
+    if (y == g.y - 2) { ry[1] = ry[2]; ry[0] = y; }
+    if (y == g.y - 3) { ry[0] = ry[1]; }
+    if (z == 0) { rz[4] = z; }
+    if (z == g.z - 1) { rz[2] = z; rz[1] = rz[4]; rz[0] = z - 2; }
+    if (z == g.z - 2) { rz[1] = rz[2]; rz[0] = z; }
+    if (z == g.z - 3) { rz[0] = rz[1]; }
+    {
+      const double m = j1[id];
+      double b[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 0, rx[q], y, z, iX) * m;
+      right[0] = weno5(b[0], b[1], b[2], b[3], b[4]);
+    }
+    {
+      const double m = j5[id];
+      double b[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 1, x, ry[q], z, iY) * m;
+      right[1] = weno5(b[0], b[1], b[2], b[3], b[4]);
+    }
+    {
+      const double m = j9[id];
+      double b[5];
+#pragma unroll
+      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 2, x, y, rz[q], iZ) * m;
+      right[2] = weno5(b[0], b[1], b[2], b[3], b[4]);
+    }
+    double r[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double v = vel[c * NG + id];
+      const double s = (double)(v > 0.0) - (double)(v < 0.0);
+      r[c] = 0.5 * (s + 1.0) * v * left[c] + 0.5 * fabs(s - 1.0) * v * right[c];
+    }
+    const double rs = r[0] + r[1] + r[2];
+    const double sv = src[id];
+    if (stage == 0) out[id] = sv - dt * rs;
+    else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (sv - dt * rs);
+    else out[id] = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+  }
+}
+
+unsigned blocks_for(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 131072) b = 131072;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+PDims padded(const Dims& d) { return PDims{d.nx + 2 * GC, d.ny + 2 * GC, d.nz + 2 * GC}; }
+
+}  // namespace
+
+size_t ls_scratch_doubles(const Dims& d) { return (size_t)10 * padded(d).N(); }
+
+int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
+                      const double* sobject, double dx, double dy, double dz,
+                      double deltamin) {
+  const Dims d = ctx->d;
+  const PDims g = padded(d);
+  const long long N = d.N(), NG = g.N();
+  double* P = ctx->ls_buf;
+  double* S = P + NG;
+  double* J1 = S + NG;
+  double* J5 = J1 + NG;
+  double* J9 = J5 + NG;
+  double* dphi = J9 + NG;  // 3*NG
+  double* D1 = dphi + 3 * NG;
+  double* D2 = D1 + NG;
+  hipStream_t st = ctx->stream;
+  const unsigned bg = blocks_for(NG);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, pres, d.nx, d.ny, d.nz, P, g);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, sobject, d.nx, d.ny, d.nz, S, g);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g);
+  // cuExtrapolation(d_press, d_sobject, d_jbn, deltaX.., NxG.., deltamin*0.5, 1)
+  // hands 1.0/deltaX to every kernel (interface.cu:373-421)
+  const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz, dt = deltamin * 0.5;
+  hipLaunchKernelGGL(k_ls_grad1, dim3(bg), dim3(256), 0, st, S, J1, J5, J9, g,
+                     iX, iY, iZ, dphi);
+  for (int it = 1; it <= 10; it++) {
+    hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 0, P, P,
+                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D1);
+    hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 1, D1, P,
+                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D2);
+    hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 2, D2, P,
+                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, P);
+  }
+  hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
+                     d.nx, d.ny, d.nz, pres);
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
+                       const double* jbn, double dx, double dy, double dz,
+                       double dt) {
+  const Dims d = ctx->d;
+  const PDims g = padded(d);
+  const long long N = d.N(), NG = g.N();
+  double* P = ctx->ls_buf;
+  double* V = P + NG;  // 3*NG
+  double* J1 = V + 3 * NG;
+  double* J5 = J1 + NG;
+  double* J9 = J5 + NG;
+  double* T1 = J9 + NG;
+  double* T2 = T1 + NG;
+  hipStream_t st = ctx->stream;
+  const unsigned bg = blocks_for(NG);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, phi, d.nx, d.ny, d.nz, P, g);
+  for (int c = 0; c < 3; c++)
+    hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, vel + c * N, d.nx, d.ny,
+                       d.nz, V + c * NG, g);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g);
+  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g);
+  const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz;
+  hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, V, J1,
+                     J5, J9, g, iX, iY, iZ, dt, T1);
+  hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, V, J1,
+                     J5, J9, g, iX, iY, iZ, dt, T2);
+  hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, V, J1,
+                     J5, J9, g, iX, iY, iZ, dt, P);
+  hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
+                     d.nx, d.ny, d.nz, phi);
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+}  // namespace emx

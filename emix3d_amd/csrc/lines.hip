@@ -1,0 +1,203 @@
+// Line-solver tables: the Thomas factors of the constant tridiagonal matrix
+// of each compact scheme, and the chunk-carry constants that let a line be
+// solved as P independent chunks plus a carry exchange.
+//
+// The matrix of every line of a direction is the same (it depends only on
+// the scheme and the line length), so the factors the reference recomputes
+// per line (solver/derivada.f90:111-122, src/Schemes/derivCompact.cu:199-212)
+// are computed once here, in FP64 on the host, and kept on the device.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "emx_internal.hpp"
+
+namespace emx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+// n >= 5 for every scheme: the C4 matrix is singular at n = 4 (its last
+// pivot bet_3 = 1 - 3*(1/3) vanishes), where the reference divides by zero.
+int scheme_min_n(int scheme) {
+  (void)scheme;
+  return 5;
+}
+
+// Coefficients a (sub), b (diag), c (super):
+//  C4 : solver/derivada.f90:10-62
+//  C6F: src/MSVSingle/derivada_incomp.f90:10-62
+//  C6C: src/Schemes/derivCompact.cu:165-189
+int host_thomas(int scheme, int n, std::vector<double>& a,
+                std::vector<double>& b, std::vector<double>& c) {
+  if (scheme < 0 || scheme > 2) return EMX_ERR_ARG;
+  if (n < scheme_min_n(scheme)) return EMX_ERR_SIZE;
+  a.assign(n, 0.0);
+  b.assign(n, 1.0);
+  c.assign(n, 0.0);
+  if (scheme == EMX_SCHEME_C4) {
+    for (int l = 1; l < n - 1; l++) a[l] = c[l] = 1. / 4.;
+    a[0] = 0.0; a[n - 1] = 3.0;
+    c[0] = 3.0; c[n - 1] = 0.0;
+  } else {
+    const double edge = scheme == EMX_SCHEME_C6F ? 2.0 : 3.0;
+    for (int l = 2; l < n - 2; l++) a[l] = c[l] = 1.0 / 3.0;
+    a[0] = 0.0; a[1] = 0.25; a[n - 2] = 0.25; a[n - 1] = edge;
+    c[0] = edge; c[1] = 0.25; c[n - 2] = 0.25; c[n - 1] = 0.0;
+  }
+  return EMX_OK;
+}
+
+namespace {
+
+struct TabKey {
+  int dev, scheme, n, C, kind;
+  bool operator<(const TabKey& o) const {
+    return std::tie(dev, scheme, n, C, kind) <
+           std::tie(o.dev, o.scheme, o.n, o.C, o.kind);
+  }
+};
+
+std::mutex g_tab_mu;
+std::map<TabKey, LineTab> g_tabs;  // device memory held until process exit
+
+int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
+  std::vector<double> a, b, c;
+  EMX_TRY(host_thomas(scheme, n, a, b, c));
+  const int P = (n + C - 1) / C;
+  if (kind == KIND_WAVE && P > 64) {
+    set_error("line of %d points needs %d chunks of %d > 64 lanes", n, P, C);
+    return EMX_ERR_SIZE;
+  }
+  if (kind == KIND_STRIDED && P > 16) {
+    set_error("line of %d points needs %d chunks of %d > 16 waves", n, P, C);
+    return EMX_ERR_SIZE;
+  }
+  std::vector<double> bet(n), gam(n + 1, 0.0), ib(n), m(n), H(n), G(n);
+  bet[0] = b[0];
+  for (int l = 1; l < n; l++) {
+    gam[l] = c[l - 1] / bet[l - 1];
+    bet[l] = b[l] - a[l] * gam[l];
+  }
+  for (int l = 0; l < n; l++) {
+    ib[l] = 1.0 / bet[l];
+    m[l] = a[l] * ib[l];
+  }
+  std::vector<double> Hend(P), Gs(P);
+  for (int ch = 0; ch < P; ch++) {
+    int s = ch * C, e = std::min(s + C, n);
+    double h = 1.0;
+    for (int l = s; l < e; l++) { h *= -m[l]; H[l] = h; }
+    for (int l = s; l < e; l++) {
+      double g = 1.0;
+      for (int t = l + 1; t <= e; t++) g *= -gam[t];  // gam[n] = 0
+      G[l] = g;
+    }
+    Hend[ch] = H[e - 1];
+    Gs[ch] = G[s];
+  }
+  std::vector<double> Wf((size_t)P * P, 0.0), Wb((size_t)P * P, 0.0);
+  for (int ch = 0; ch < P; ch++) {
+    for (int t = 0; t < ch; t++) {
+      double w = 1.0;
+      for (int u = t + 1; u <= ch - 1; u++) w *= Hend[u];
+      Wf[(size_t)ch * P + t] = w;
+    }
+    for (int t = ch + 1; t < P; t++) {
+      double w = 1.0;
+      for (int u = ch + 1; u <= t - 1; u++) w *= Gs[u];
+      Wb[(size_t)ch * P + t] = w;
+    }
+  }
+  std::vector<double> Af(6 * 64, 0.0), Ab(6 * 64, 0.0);
+  for (int k = 0; k < 6; k++) {
+    int d = 1 << k;
+    for (int ch = 0; ch < 64; ch++) {
+      if (ch < P && ch >= d) {
+        double w = 1.0;
+        for (int u = ch - d + 1; u <= ch; u++) w *= Hend[u];
+        Af[k * 64 + ch] = w;
+      }
+      if (ch < P && ch + d < P) {
+        double w = 1.0;
+        for (int u = ch; u <= ch + d - 1; u++) w *= Gs[u];
+        Ab[k * 64 + ch] = w;
+      }
+    }
+  }
+  // one device block: ib m gam H G Wf Wb Af Ab
+  size_t off[10];
+  off[0] = 0;
+  off[1] = off[0] + n;          // m
+  off[2] = off[1] + n;          // gam (n+1)
+  off[3] = off[2] + n + 1;      // H
+  off[4] = off[3] + n;          // G
+  off[5] = off[4] + n;          // Wf
+  off[6] = off[5] + (size_t)P * P;  // Wb
+  off[7] = off[6] + (size_t)P * P;  // Af
+  off[8] = off[7] + 6 * 64;         // Ab
+  off[9] = off[8] + 6 * 64;
+  std::vector<double> host(off[9]);
+  memcpy(&host[off[0]], ib.data(), n * sizeof(double));
+  memcpy(&host[off[1]], m.data(), n * sizeof(double));
+  memcpy(&host[off[2]], gam.data(), (n + 1) * sizeof(double));
+  memcpy(&host[off[3]], H.data(), n * sizeof(double));
+  memcpy(&host[off[4]], G.data(), n * sizeof(double));
+  memcpy(&host[off[5]], Wf.data(), (size_t)P * P * sizeof(double));
+  memcpy(&host[off[6]], Wb.data(), (size_t)P * P * sizeof(double));
+  memcpy(&host[off[7]], Af.data(), 6 * 64 * sizeof(double));
+  memcpy(&host[off[8]], Ab.data(), 6 * 64 * sizeof(double));
+  double* dev = nullptr;
+  if (hipMalloc(&dev, off[9] * sizeof(double)) != hipSuccess) {
+    set_error("hipMalloc of a %zu-byte line table failed", off[9] * sizeof(double));
+    return EMX_ERR_NOMEM;
+  }
+  EMX_TRY_HIP(hipMemcpy(dev, host.data(), off[9] * sizeof(double),
+                        hipMemcpyHostToDevice));
+  t->n = n;
+  t->C = C;
+  t->P = P;
+  t->kind = kind;
+  t->ib = dev + off[0];
+  t->m = dev + off[1];
+  t->gam = dev + off[2];
+  t->H = dev + off[3];
+  t->G = dev + off[4];
+  t->Wf = dev + off[5];
+  t->Wb = dev + off[6];
+  t->Af = dev + off[7];
+  t->Ab = dev + off[8];
+  return EMX_OK;
+}
+
+}  // namespace
+
+int get_line_tab(int scheme, int n, int C, int kind, LineTab* out) {
+  int dev = 0;
+  EMX_TRY_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  TabKey key{dev, scheme, n, C, kind};
+  auto it = g_tabs.find(key);
+  if (it != g_tabs.end()) {
+    *out = it->second;
+    return EMX_OK;
+  }
+  LineTab t{};
+  EMX_TRY(build_tab(scheme, n, C, kind, &t));
+  g_tabs[key] = t;
+  *out = t;
+  return EMX_OK;
+}
+
+}  // namespace emx

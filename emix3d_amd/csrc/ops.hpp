@@ -1,0 +1,42 @@
+// Host-side operator launchers shared by the C ABI (capi.hip).
+#pragma once
+#include "emx_internal.hpp"
+
+namespace emx {
+
+struct FluxPtrs {
+  const double* um;
+  const double* u;
+  const double* pres;
+  const double* vis;
+  const double* jbn;
+  const double* dvel;
+};
+
+const char* last_error();
+
+int op_deriv(emx_ctx* ctx, int dir, const double* f, double* df, double ih);
+int op_grad(emx_ctx* ctx, const double* u, const double* temp, double* dvel,
+            double* dtemp, const double ih[3]);
+int op_flux(emx_ctx* ctx, int neq, const FluxPtrs& fp, double* flux);
+int op_divergence(emx_ctx* ctx, const double* flux, const double* jbn,
+                  const double ih[3], double* rs, double* rsv);
+int op_flux_div(emx_ctx* ctx, int neq, const FluxPtrs& fp, const double ih[3],
+                double* rs, double* rsv);
+int op_metrics(emx_ctx* ctx, const double* x, double* jbn, int* bad_dev);
+int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
+                    double* dmins, double* blockmax, int nblocks);
+int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
+            const double* jbn, const double* sgd, double c4, double* amut,
+            double* vis, double* vis5);
+
+// level set (levelset.hip); scratch is ctx->ls_buf
+size_t ls_scratch_doubles(const Dims& d);
+int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
+                      const double* sobject, double dx, double dy, double dz,
+                      double deltamin);
+int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
+                       const double* jbn, double dx, double dy, double dz,
+                       double dt);
+
+}  // namespace emx

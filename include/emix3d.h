@@ -1,0 +1,182 @@
+/*
+ * emix3d.h -- C ABI of libemix3d_hip.so, the MI355X (gfx950) implementation
+ * of eMix3d's per-RK-stage spatial operator.
+ *
+ * Data: FP64, Fortran column-major.  Element (i,j,k,c) of an (nx,ny,nz,nc)
+ * array is at i + nx*(j + ny*(k + nz*c)), 0-based; `N` below is nx*ny*nz.
+ * Component numbering follows the reference:
+ *   jbn  (11 comps): 1..9 inverse metrics (1=xi_x, 5=eta_y, 9=zeta_z),
+ *                    10 = J, 11 = 1/J          (solver/jacob_malla.f90)
+ *   dvel (9 comps):  u_xi, v_xi, w_xi, u_eta, v_eta, w_eta, u_zeta, v_zeta,
+ *                    w_zeta                    (solver/deriv_vel.f90:17-123)
+ *   um   (5 comps):  conservative variables of RUNGEK
+ *   flux (6 comps):  e, f, g, ev, fv, gv       (solver/flujos_compact.f90)
+ *   sgd  (7 comps):  dxyzsgd, dxpsgd, dxmsgd, dypsgd, dymsgd, dzpsgd, dzmsgd
+ *                                              (solver/sgdm_incomp.f90:2-114)
+ * `ih*` arguments are inverse computational spacings (the Fortran
+ * `deltax` = float(nx-1), src/MSVSingle/atl_incomp.f90:30-32).
+ *
+ * Two families of entry points:
+ *   - emx_* take an emx_ctx and DEVICE pointers; they are asynchronous on
+ *     the context's stream, allocate nothing, and can be captured in a HIP
+ *     graph.  They return EMX_OK or a negative status; emx_last_error()
+ *     describes the last failure of the calling thread.
+ *   - the reference's own bind(C) symbols (cudaDeriv, cuDivergence,
+ *     advect_CUDA) with their exact signatures, taking HOST pointers and
+ *     returning after the results are back in host memory, plus host
+ *     variants of the two routines the reference runs in Fortran.
+ */
+#ifndef EMIX3D_H
+#define EMIX3D_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EMX_VERSION 1
+
+/* compact schemes:
+ *  C4 : 4th-order Pade alpha=1/4, 4th-order alpha=3 closure
+ *       (solver/derivada.f90:2-125)
+ *  C6F: 6th-order alpha=1/3, 4th-order rows 2/n-1, 3rd-order alpha=2
+ *       closure (src/MSVSingle/derivada_incomp.f90:2-90,169-201)
+ *  C6C: 6th-order alpha=1/3, 4th-order rows 1/N-2, 4th-order alpha=3
+ *       closure (src/Schemes/derivCompact.cu:83-215) */
+#define EMX_SCHEME_C4 0
+#define EMX_SCHEME_C6F 1
+#define EMX_SCHEME_C6C 2
+
+/* status codes */
+#define EMX_OK 0
+#define EMX_ERR_ARG (-1)    /* bad argument (null pointer, bad enum)      */
+#define EMX_ERR_HIP (-2)    /* HIP runtime error                          */
+#define EMX_ERR_SIZE (-3)   /* grid too small / too large for the scheme  */
+#define EMX_ERR_METRIC (-4) /* degenerate metric (|J| <= 1e-20)           */
+#define EMX_ERR_NOMEM (-5)  /* device allocation failed                   */
+
+typedef struct emx_ctx emx_ctx;
+
+/* ---- library and context ------------------------------------------------ */
+
+/* Text of the last failure on the calling thread ("" if none). */
+const char* emx_last_error(void);
+int emx_version(void);
+
+/* Create a context for an nx*ny*nz grid and a scheme, on the current HIP
+ * device, launching on `stream` (a hipStream_t; NULL = default stream).
+ * Allocates the line-solver tables and 6*N doubles of scratch once. */
+int emx_create(emx_ctx** ctx, int nx, int ny, int nz, int scheme,
+               void* stream);
+int emx_destroy(emx_ctx* ctx);
+int emx_set_stream(emx_ctx* ctx, void* stream);
+/* Line-solver launch shape of direction dir (0=xi,1=eta,2=zeta):
+ * kind 0 = one wavefront per line, kind 1 = 64 lines per workgroup with the
+ * line split over waves; C = points per work-item.  Used by the autotuner. */
+int emx_set_plan(emx_ctx* ctx, int dir, int kind, int C);
+int emx_get_plan(const emx_ctx* ctx, int dir, int* kind, int* C);
+int emx_sync(emx_ctx* ctx);
+
+/* ---- device-pointer operator entry points ------------------------------- */
+
+/* One compact first derivative along dir: one call of derivnper per line
+ * (solver/derivada.f90:95-125) / cu::cs::dF{X,Y,Z}
+ * (src/Schemes/derivCompact.cu:222-409).  df must not alias f. */
+int emx_deriv(emx_ctx* ctx, int dir, const double* f, double* df,
+              double inv_h);
+
+/* Velocity (3 comps) and temperature gradients in computational space:
+ * DERIV_VEL (solver/deriv_vel.f90:1-203) / cudaDeriv
+ * (solver/cuDeriv.cu:529-654).  temp/dtemp may both be NULL. */
+int emx_grad(emx_ctx* ctx, const double* u, const double* temp, double* dvel,
+             double* dtemp, double ihx, double ihy, double ihz);
+
+/* Grid metrics from coordinates x (3 comps): JACOBEANO
+ * (solver/jacob_malla.f90:1-171), with the context's scheme and
+ * ih = n-1 per direction.  Returns EMX_ERR_METRIC if any |J| <= 1e-20. */
+int emx_metrics(emx_ctx* ctx, const double* x, double* jbn);
+
+/* SGS geometry: INISGDM (solver/sgdm_incomp.f90:2-114).  dmins may be
+ * NULL; *deltamax (host) receives max spacing when non-NULL (synchronises). */
+int emx_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
+                     double* dmins, double* deltamax);
+
+/* Structure-function eddy viscosity and total viscosity: viscosidad_t
+ * (solver/sgdm_incomp.f90:117-395) + viscosidad
+ * (solver/viscosidad_incomp.f90:1-23).  vis5 may be NULL. */
+int emx_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
+             const double* jbn, const double* sgd, double c4, double* amut,
+             double* vis, double* vis5);
+
+/* Pointwise fluxes of equation neq (1..5): flujos(neq)
+ * (solver/flujos_compact.f90:1-220).  flux: 6 comps. */
+int emx_flux(emx_ctx* ctx, int neq, const double* um, const double* u,
+             const double* pres, const double* vis, const double* jbn,
+             const double* dvel, double* flux);
+
+/* Flux divergence: divergencia(neq) (solver/divergencia.f90:1-154) /
+ * cuDivergence (solver/cuDivergence.cu:107-189).  flux is not modified. */
+int emx_divergence(emx_ctx* ctx, const double* flux, const double* jbn,
+                   double ihx, double ihy, double ihz, double* rs,
+                   double* rsv);
+
+/* flujos(neq) followed by divergencia(neq) without materialising the six
+ * flux fields (the pair RUNGEK calls per equation, solver/rungek_ico.f90:
+ * 83-95). */
+int emx_flux_div(emx_ctx* ctx, int neq, const double* um, const double* u,
+                 const double* pres, const double* vis, const double* jbn,
+                 const double* dvel, double ihx, double ihy, double ihz,
+                 double* rs, double* rsv);
+
+/* Pressure extrapolation into the solid (sobject > 0): advect_CUDA
+ * (src/LevelSet/lsTransportEc.cu:24-197 -> interface.cu:341-436).
+ * dx,dy,dz are the computational spacings the reference passes
+ * (1/deltax).  pres is updated in place. */
+int emx_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
+                       const double* sobject, double dx, double dy,
+                       double dz, double deltamin);
+
+/* One SSP-RK3 step of WENO5 level-set advection, cuAdvectLsJB
+ * (src/LevelSet/lsTransportEc.cu:201-418) on the 5-cell mirror padding
+ * that advect_CUDA builds (lsTransportEc.cu:100-166).  phi in place;
+ * vel: 3 comps. */
+int emx_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
+                        const double* jbn, double dx, double dy, double dz,
+                        double dt);
+
+/* ---- host-pointer drop-ins with the reference's bind(C) signatures ------ */
+
+/* solver/deriv_vel_cuda.f90:19-39 -> solver/cuDeriv.cu:529-540.
+ * amut, dxyzsgd and jbn are accepted and unused, as in the reference.
+ * Scheme C6C.  On error prints and exits(-1), as the reference does. */
+void cudaDeriv(double* dvel, double* dtemp, double* u, double* temp,
+               double* amut, double* dxyzsgd, double* jbn, double dx,
+               double dy, double dz, int Nx, int Ny, int Nz);
+
+/* solver/divergencia_cuda.f90:21-42 -> solver/cuDivergence.cu:107-119.
+ * Scheme C6C.  e..gv are not modified. */
+void cuDivergence(double* rs, double* rsv, double* e, double* f, double* g,
+                  double* ev, double* fv, double* gv, double* jbn, double dx,
+                  double dy, double dz, int Nx, int Ny, int Nz);
+
+/* include/LevelSet/interfaceAdvect.h:3-16 -> lsTransportEc.cu:24-35. */
+void advect_CUDA(double* pres, double* jbn, double* sobject, double dx,
+                 double dy, double dz, int Nx, int Ny, int Nz,
+                 double deltamin);
+
+/* Host-pointer forms of the two hot-path routines the reference runs in
+ * Fortran on the CPU, for the Fortran shim: flujos(neq)
+ * (solver/flujos_compact.f90:1-220) and viscosidad_t+viscosidad
+ * (solver/sgdm_incomp.f90:117-395, solver/viscosidad_incomp.f90:1-23).
+ * Return a status like the emx_* functions. */
+int emx_host_flujos(int neq, const double* um, const double* u,
+                    const double* pres, const double* vis, const double* jbn,
+                    const double* dvel, double* e, double* f, double* g,
+                    double* ev, double* fv, double* gv, int Nx, int Ny,
+                    int Nz);
+int emx_host_sgdm(const double* u, const double* dvel, const double* jbn,
+                  const double* sgd, double c4, double* amut, double* vis,
+                  double* vis5, int Nx, int Ny, int Nz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

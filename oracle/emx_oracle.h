@@ -1,0 +1,123 @@
+/*
+ * emx_oracle.h -- CPU oracle for the eMix3d per-RK-stage spatial operator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a serial C restatement of the
+ * reference algorithms, used as the checker by tests/, by
+ * __graft_entry__.smoke() and by the cpu_baseline leg of bench.py.  The
+ * product path (emix3d_amd, libemix3d_hip.so) never links or calls it.
+ *
+ * Conventions (identical to the reference, solver/mix3d.mk:3 and
+ * src/Engine/cuArray.cu:119-124):
+ *   - FP64 everywhere;
+ *   - arrays are Fortran column-major (nx,ny,nz[,c]); element (i,j,k,c),
+ *     0-based, sits at i + nx*(j + ny*(k + nz*c));
+ *   - inv_h is the inverse grid spacing in computational space, the
+ *     Fortran `delta` = float(N-1) of src/MSVSingle/atl_incomp.f90:30-32.
+ *
+ * Parity pins (what checks each function; details in tests/):
+ *   orc_deriv_line C4/C6F, orc_jacobeano, orc_deriv_vel, orc_flujos,
+ *   orc_divergencia, orc_inisgdm, orc_sgdm:
+ *       golden vectors from the reference Fortran built by flang
+ *       (oracle/_ref), plus closed forms (polynomial exactness, dense solve).
+ *   orc_deriv_line C6C: CUDA-only in the reference; pinned by polynomial
+ *       exactness (degree <= 4 on every row) and a dense solve of the
+ *       matrices written in src/Schemes/derivCompact.cu:83-215.
+ *   orc_ls_extrapolate, orc_ls_advect_weno5: CUDA-only in the reference;
+ *       pinned by invariants and closed forms (constant preservation,
+ *       untouched fluid points, WENO5 exactness on cubics and 5th-order
+ *       convergence, 1-D upwind reduction).
+ *   orc_dynsmag: no reference implementation exists (the reference's
+ *       model is the structure-function sgdm); pinned only by closed forms.
+ */
+#ifndef EMX_ORACLE_H
+#define EMX_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_C4 = 0, ORC_C6F = 1, ORC_C6C = 2 };
+
+/* Tridiagonal coefficients (a: sub, b: diag, c: super) of a scheme, n rows. */
+int orc_coeffs(int scheme, int n, double *a, double *b, double *c);
+
+/* Compact first derivative of one line of n points read with stride `is`
+ * from f, written with stride `os` to out (out may alias f).  Returns 0, or
+ * -1 when n is below the scheme's minimum. */
+int orc_deriv_line(int scheme, int n, const double *f, long is, double inv_h,
+                   double *out, long os);
+
+/* Compact derivative of a whole field along dir (0=xi, 1=eta, 2=zeta). */
+int orc_deriv3d(int scheme, int dir, int nx, int ny, int nz, const double *f,
+                double inv_h, double *df);
+
+/* Grid metrics, solver/jacob_malla.f90:1-171 (x: (nx,ny,nz,3); jbn: 11). */
+int orc_jacobeano(int scheme, int nx, int ny, int nz, const double *x,
+                  double *jbn);
+
+/* Velocity/temperature gradients, solver/deriv_vel.f90:1-161
+ * (u: 3 comps, temp: 1, dvel: 9, dtemp: 3; temp/dtemp may be NULL). */
+int orc_deriv_vel(int scheme, int nx, int ny, int nz, const double *u,
+                  const double *temp, double *dvel, double *dtemp,
+                  double ihx, double ihy, double ihz);
+
+/* Pointwise fluxes, solver/flujos_compact.f90:1-220.
+ * flux: 6 comps e,f,g,ev,fv,gv. */
+int orc_flujos(int neq, int nx, int ny, int nz, const double *um,
+               const double *u, const double *pres, const double *vis,
+               const double *jbn, const double *dvel, double *flux);
+
+/* Flux divergence, solver/divergencia.f90:1-154 (flux: 6 comps, not
+ * modified). */
+int orc_divergencia(int scheme, int nx, int ny, int nz, const double *flux,
+                    const double *jbn, double ihx, double ihy, double ihz,
+                    double *rs, double *rsv);
+
+/* SGS geometry, solver/sgdm_incomp.f90:2-114.  sgd: 7 comps
+ * (dxyzsgd, dxp, dxm, dyp, dym, dzp, dzm); dmins may be NULL. */
+int orc_inisgdm(int nx, int ny, int nz, const double *x, double *sgd,
+                double *dmins, double *deltamax);
+
+/* Structure-function eddy viscosity + viscosidad,
+ * solver/sgdm_incomp.f90:117-395 and solver/viscosidad_incomp.f90:1-23.
+ * vis5 may be NULL.  work: 3*N doubles of scratch (vorticity). */
+int orc_sgdm(int nx, int ny, int nz, const double *u, const double *dvel,
+             const double *jbn, const double *sgd, double c4, double *amut,
+             double *vis, double *vis5);
+
+/* Pressure extrapolation into the solid, the advect_CUDA path of
+ * src/LevelSet/lsTransportEc.cu:24-197 -> src/LevelSet/interface.cu:341-436.
+ * dx,dy,dz are the grid spacings (the reference passes 1/deltax);
+ * pres is updated in place. */
+int orc_ls_extrapolate(int nx, int ny, int nz, double *pres, const double *jbn,
+                       const double *sobject, double dx, double dy, double dz,
+                       double deltamin);
+
+/* One SSP-RK3 step of WENO5 level-set advection on ghost-padded arrays,
+ * src/LevelSet/lsTransportEc.cu:201-418 (cuAdvectLsJB).  phi, vel (3 comps)
+ * and jbn (11 comps, only 1/5/9 read) have padded dims (ngx,ngy,ngz). */
+int orc_ls_advect_weno5_padded(int ngx, int ngy, int ngz, double *phi,
+                               const double *vel, const double *jbn,
+                               double dx, double dy, double dz, double dt);
+
+/* Same step on unpadded arrays: mirror-pad by 5 cells the way advect_CUDA
+ * pads its inputs (lsTransportEc.cu:100-166), step, copy the interior back. */
+int orc_ls_advect_weno5(int nx, int ny, int nz, double *phi, const double *vel,
+                        const double *jbn, double dx, double dy, double dz,
+                        double dt);
+
+/* Even mirror padding by g cells, as cuSwapToGhost followed by
+ * cuGhostCellsMirror3d{X,Y,Z} (src/LevelSet/lsTools.cu:482-585,772-796). */
+void orc_pad_mirror(int nx, int ny, int nz, int g, const double *in,
+                    double *out);
+
+/* Dynamic Smagorinsky (Germano-Lilly) eddy viscosity with a box test filter
+ * of ratio 2.  Not in the reference (see header); inputs are the velocity
+ * and its physical-space gradient g[3*a+b] = du_a/dx_b (9 comps), and the
+ * grid filter width delta (one value per point).  Clipped at nu_t >= 0. */
+int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gphys,
+                const double *delta, double *nut);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

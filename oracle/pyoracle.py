@@ -1,0 +1,287 @@
+"""ctypes access to the CPU oracle and to the reference Fortran build.
+
+TEST INFRASTRUCTURE ONLY -- imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, never by the product package.
+
+Array convention (same memory as the reference's Fortran column-major
+(nx,ny,nz[,c]) arrays): numpy C-contiguous float64 arrays of shape
+(nz, ny, nx) for one field and (c, nz, ny, nx) for c components.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+C4, C6F, C6C = 0, 1, 2
+SCHEMES = {"c4": C4, "c6f": C6F, "c6c": C6C}
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_L = ctypes.c_long
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "float64 C-contiguous"
+    return a.ctypes.data_as(_P)
+
+
+def build():
+    """Compile the C restatement (and the reference, where present)."""
+    subprocess.run(["make", "-s", "-C", HERE, "libemx_oracle.so"], check=True)
+    subprocess.run([os.path.join(HERE, "build_ref.sh")], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "libemx_oracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        sig = {
+            "orc_coeffs": [_I, _I, _P, _P, _P],
+            "orc_deriv_line": [_I, _I, _P, _L, _D, _P, _L],
+            "orc_deriv3d": [_I, _I, _I, _I, _I, _P, _D, _P],
+            "orc_jacobeano": [_I, _I, _I, _I, _P, _P],
+            "orc_deriv_vel": [_I, _I, _I, _I, _P, _P, _P, _P, _D, _D, _D],
+            "orc_flujos": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P],
+            "orc_divergencia": [_I, _I, _I, _I, _P, _P, _D, _D, _D, _P, _P],
+            "orc_inisgdm": [_I, _I, _I, _P, _P, _P, _P],
+            "orc_sgdm": [_I, _I, _I, _P, _P, _P, _P, _D, _P, _P, _P],
+            "orc_ls_extrapolate": [_I, _I, _I, _P, _P, _P, _D, _D, _D, _D],
+            "orc_ls_advect_weno5_padded": [_I, _I, _I, _P, _P, _P, _D, _D, _D, _D],
+            "orc_ls_advect_weno5": [_I, _I, _I, _P, _P, _P, _D, _D, _D, _D],
+            "orc_pad_mirror": [_I, _I, _I, _I, _P, _P],
+            "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = None if name == "orc_pad_mirror" else _I
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ValueError(f"oracle {what} failed with status {rc}")
+
+
+def dims(a):
+    nz, ny, nx = a.shape[-3:]
+    return nx, ny, nz
+
+
+def coeffs(scheme, n):
+    a, b, c = (np.zeros(n) for _ in range(3))
+    _check(lib().orc_coeffs(scheme, n, _ptr(a), _ptr(b), _ptr(c)), "coeffs")
+    return a, b, c
+
+
+def deriv_line(scheme, f, inv_h):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty_like(f)
+    _check(lib().orc_deriv_line(scheme, f.size, _ptr(f), 1, inv_h, _ptr(out), 1), "deriv_line")
+    return out
+
+
+def deriv3d(scheme, d, f, inv_h):
+    nx, ny, nz = dims(f)
+    out = np.empty_like(f)
+    _check(lib().orc_deriv3d(scheme, d, nx, ny, nz, _ptr(f), inv_h, _ptr(out)), "deriv3d")
+    return out
+
+
+def jacobeano(scheme, x):
+    nx, ny, nz = dims(x)
+    jbn = np.zeros((11, nz, ny, nx))
+    _check(lib().orc_jacobeano(scheme, nx, ny, nz, _ptr(x), _ptr(jbn)), "jacobeano")
+    return jbn
+
+
+def deriv_vel(scheme, u, temp, ih):
+    nx, ny, nz = dims(u)
+    dvel = np.empty((9, nz, ny, nx))
+    dtemp = np.empty((3, nz, ny, nx))
+    _check(lib().orc_deriv_vel(scheme, nx, ny, nz, _ptr(u), _ptr(temp), _ptr(dvel),
+                               _ptr(dtemp), ih[0], ih[1], ih[2]), "deriv_vel")
+    return dvel, dtemp
+
+
+def flujos(neq, um, u, pres, vis, jbn, dvel):
+    nx, ny, nz = dims(u)
+    flux = np.empty((6, nz, ny, nx))
+    _check(lib().orc_flujos(neq, nx, ny, nz, _ptr(um), _ptr(u), _ptr(pres), _ptr(vis),
+                            _ptr(jbn), _ptr(dvel), _ptr(flux)), "flujos")
+    return flux
+
+
+def divergencia(scheme, flux, jbn, ih):
+    nx, ny, nz = dims(flux)
+    rs = np.empty((nz, ny, nx))
+    rsv = np.empty((nz, ny, nx))
+    _check(lib().orc_divergencia(scheme, nx, ny, nz, _ptr(flux), _ptr(jbn), ih[0], ih[1],
+                                 ih[2], _ptr(rs), _ptr(rsv)), "divergencia")
+    return rs, rsv
+
+
+def inisgdm(x):
+    nx, ny, nz = dims(x)
+    sgd = np.empty((7, nz, ny, nx))
+    dmins = np.empty((nz, ny, nx))
+    dmax = ctypes.c_double(0.0)
+    _check(lib().orc_inisgdm(nx, ny, nz, _ptr(x), _ptr(sgd), _ptr(dmins),
+                             ctypes.byref(dmax)), "inisgdm")
+    return sgd, dmins, dmax.value
+
+
+def sgdm(u, dvel, jbn, sgd, c4):
+    nx, ny, nz = dims(u)
+    amut, vis, vis5 = (np.empty((nz, ny, nx)) for _ in range(3))
+    _check(lib().orc_sgdm(nx, ny, nz, _ptr(u), _ptr(dvel), _ptr(jbn), _ptr(sgd), c4,
+                          _ptr(amut), _ptr(vis), _ptr(vis5)), "sgdm")
+    return amut, vis, vis5
+
+
+def ls_extrapolate(pres, jbn, sobject, dxyz, deltamin):
+    nx, ny, nz = dims(pres)
+    out = np.array(pres, dtype=np.float64, copy=True)
+    _check(lib().orc_ls_extrapolate(nx, ny, nz, _ptr(out), _ptr(jbn), _ptr(sobject),
+                                    dxyz[0], dxyz[1], dxyz[2], deltamin), "ls_extrapolate")
+    return out
+
+
+def ls_advect_weno5(phi, vel, jbn, dxyz, dt):
+    nx, ny, nz = dims(phi)
+    out = np.array(phi, dtype=np.float64, copy=True)
+    _check(lib().orc_ls_advect_weno5(nx, ny, nz, _ptr(out), _ptr(vel), _ptr(jbn),
+                                     dxyz[0], dxyz[1], dxyz[2], dt), "ls_advect_weno5")
+    return out
+
+
+def ls_advect_weno5_padded(phi, vel, jbn, dxyz, dt):
+    nx, ny, nz = dims(phi)
+    out = np.array(phi, dtype=np.float64, copy=True)
+    _check(lib().orc_ls_advect_weno5_padded(nx, ny, nz, _ptr(out), _ptr(vel), _ptr(jbn),
+                                            dxyz[0], dxyz[1], dxyz[2], dt), "weno5_padded")
+    return out
+
+
+def pad_mirror(a, g=5):
+    nx, ny, nz = dims(a)
+    out = np.empty((nz + 2 * g, ny + 2 * g, nx + 2 * g))
+    lib().orc_pad_mirror(nx, ny, nz, g, _ptr(np.ascontiguousarray(a)), _ptr(out))
+    return out
+
+
+def dynsmag(u, gphys, delta):
+    nx, ny, nz = dims(u)
+    nut = np.empty((nz, ny, nx))
+    _check(lib().orc_dynsmag(nx, ny, nz, _ptr(u), _ptr(gphys), _ptr(delta), _ptr(nut)),
+           "dynsmag")
+    return nut
+
+
+# --------------------------------------------------------------------------
+# The reference Fortran, built by build_ref.sh into _ref/.
+
+class RefFortran:
+    """The reference's own Fortran hot path (flang build), one scheme per
+    library: 'c4' = solver/derivada.f90, 'c6f' = derivada_incomp.f90.
+    The module state is global, so one instance per library."""
+
+    _cache = {}
+
+    def __new__(cls, scheme):
+        if scheme not in cls._cache:
+            obj = super().__new__(cls)
+            path = os.path.join(HERE, "_ref", f"libemx_ref_{scheme}.so")
+            if not os.path.exists(path):
+                raise FileNotFoundError(path)
+            obj.L = ctypes.CDLL(path)
+            obj.scheme = scheme
+            obj.shape = None
+            cls._cache[scheme] = obj
+        return cls._cache[scheme]
+
+    @staticmethod
+    def available(scheme="c4"):
+        return os.path.exists(os.path.join(HERE, "_ref", f"libemx_ref_{scheme}.so"))
+
+    def init(self, nx, ny, nz):
+        if self.shape != (nx, ny, nz):
+            self.L.ref_init(_I(nx), _I(ny), _I(nz))
+            self.shape = (nx, ny, nz)
+
+    def derivnper(self, line, idir):
+        line = np.array(line, dtype=np.float64, copy=True)
+        self.L.ref_derivnper(_I(line.size), _ptr(line), _I(idir))
+        return line
+
+    def jacobeano(self, x):
+        self.init(*dims(x))
+        jbn = np.empty((11,) + x.shape[1:])
+        self.L.ref_jacobeano(_ptr(x), _ptr(jbn))
+        return jbn
+
+    def deriv_vel(self, u, temp):
+        self.init(*dims(u))
+        dvel = np.empty((9,) + u.shape[1:])
+        dtemp = np.empty((3,) + u.shape[1:])
+        self.L.ref_deriv_vel(_ptr(u), _ptr(temp), _ptr(dvel), _ptr(dtemp))
+        return dvel, dtemp
+
+    def flujos(self, neq, um, u, pres, vis, jbn, dvel):
+        self.init(*dims(u))
+        flux = np.empty((6,) + u.shape[1:])
+        self.L.ref_flujos(_I(neq), _ptr(um), _ptr(u), _ptr(pres), _ptr(vis), _ptr(jbn),
+                          _ptr(dvel), _ptr(flux))
+        return flux
+
+    def divergencia(self, neq, flux, jbn):
+        self.init(*dims(flux))
+        rs = np.empty(flux.shape[1:])
+        rsv = np.empty(flux.shape[1:])
+        self.L.ref_divergencia(_I(neq), _ptr(flux), _ptr(jbn), _ptr(rs), _ptr(rsv))
+        return rs, rsv
+
+    def inisgdm(self, x):
+        self.init(*dims(x))
+        sgd = np.empty((7,) + x.shape[1:])
+        dmins = np.empty(x.shape[1:])
+        dmax = ctypes.c_double(0.0)
+        self.L.ref_inisgdm(_ptr(x), _ptr(sgd), _ptr(dmins), ctypes.byref(dmax))
+        return sgd, dmins, dmax.value
+
+    def sgdm(self, u, dvel, jbn, sgd, c4):
+        self.init(*dims(u))
+        amut, vis, vis5 = (np.empty(u.shape[1:]) for _ in range(3))
+        self.L.ref_sgdm(_ptr(u), _ptr(dvel), _ptr(jbn), _ptr(sgd), _D(c4), _ptr(amut),
+                        _ptr(vis), _ptr(vis5))
+        return amut, vis, vis5
+
+    # whole RK-stage operator (RUNGEK sub-step body, for the CPU baseline)
+    def stage_load(self, um, u, temp, pres, jbn, sgd, c4):
+        self.init(*dims(u))
+        self._keep = (um, u, temp, pres, jbn, sgd)
+        self.L.ref_stage_load(_ptr(um), _ptr(u), _ptr(temp), _ptr(pres), _ptr(jbn),
+                              _ptr(sgd), _D(c4))
+
+    def stage_run(self):
+        self.L.ref_stage_run()
+
+    def stage_fetch(self):
+        shp = self._keep[1].shape[1:]
+        rs = np.empty((5,) + shp)
+        rsv = np.empty((5,) + shp)
+        amut, vis = np.empty(shp), np.empty(shp)
+        dvel = np.empty((9,) + shp)
+        self.L.ref_stage_fetch(_ptr(rs), _ptr(rsv), _ptr(amut), _ptr(vis), _ptr(dvel))
+        return rs, rsv, amut, vis, dvel

@@ -1,0 +1,82 @@
+"""Write the golden vectors of the reference Fortran hot path.
+
+Runs the reference's own Fortran (built from /root/reference by
+oracle/build_ref.sh into oracle/_ref/) on seeded inputs and stores inputs
+and outputs as .npz fixtures next to this script.  Run in the build
+container only (the reference tree is not shipped):
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from emix3d_amd import fields  # noqa: E402
+from oracle.pyoracle import RefFortran  # noqa: E402
+
+GRIDS = [
+    # name, (nx, ny, nz), curvilinear, turbulent, seed
+    ("g12x10x9_tanh", (12, 10, 9), True, True, 20261202),
+    ("g17x13x11_unif", (17, 13, 11), False, False, 20261203),
+]
+LINE_N = {"c4": [4, 5, 6, 7, 8, 16, 33], "c6f": [5, 6, 7, 8, 16, 33]}
+
+
+def np64(t):
+    return np.ascontiguousarray(t.cpu().numpy(), dtype=np.float64)
+
+
+def make_lines(scheme):
+    ref = RefFortran(scheme)
+    rng = np.random.default_rng(7)
+    out = {}
+    for n in LINE_N[scheme]:
+        ref.init(n, n, n)
+        f = rng.standard_normal(n)
+        out[f"in_{n}"] = f
+        for d in range(3):
+            out[f"out_{n}_d{d}"] = ref.derivnper(f, d + 1)
+    return out
+
+
+def make_grid(scheme, name, shape, curv, turb, seed):
+    ref = RefFortran(scheme)
+    nx, ny, nz = shape
+    flds = fields.stage_fields(nx, ny, nz, seed, curvilinear=curv, turbulent=turb)
+    x, u, temp, pres, um = (np64(flds[k]) for k in ("x", "u", "temp", "pres", "um"))
+    ref.init(nx, ny, nz)
+    jbn = ref.jacobeano(x)
+    dvel, dtemp = ref.deriv_vel(u, temp)
+    sgd, dmins, dmax = ref.inisgdm(x)
+    c4 = 1.0 / fields.REYNOLDS
+    amut, vis, vis5 = ref.sgdm(u, dvel, jbn, sgd, c4)
+    out = dict(x=x, u=u, temp=temp, pres=pres, um=um, jbn=jbn, dvel=dvel, dtemp=dtemp,
+               sgd=sgd, dmins=dmins, deltamax=np.array(dmax), c4=np.array(c4),
+               amut=amut, vis=vis, vis5=vis5)
+    for neq in range(1, 6):
+        flux = ref.flujos(neq, um, u, pres, vis, jbn, dvel)
+        rs, rsv = ref.divergencia(neq, flux, jbn)
+        out[f"flux{neq}"] = flux
+        out[f"rs{neq}"] = rs
+        out[f"rsv{neq}"] = rsv
+    return out
+
+
+def main():
+    for scheme in ("c4", "c6f"):
+        if not RefFortran.available(scheme):
+            raise SystemExit("oracle/_ref is not built: run `make -C oracle ref`")
+        np.savez_compressed(os.path.join(HERE, f"ref_{scheme}_lines.npz"), **make_lines(scheme))
+        for name, shape, curv, turb, seed in GRIDS:
+            np.savez_compressed(os.path.join(HERE, f"ref_{scheme}_{name}.npz"),
+                                **make_grid(scheme, name, shape, curv, turb, seed))
+    print("golden vectors written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
