@@ -174,12 +174,18 @@ def cpu_baseline(f, jbn, sgd, c4, work, n, scheme):
     hj = np.ascontiguousarray(jbn.cpu().numpy())
     hs = np.ascontiguousarray(sgd.cpu().numpy())
     ref = RefFortran(lib_scheme)
+    ref.init(n, n, n)
+    # INISGDM sets the model constants (isgm, amu0, cutoff) of the run;
+    # its geometry fields must equal the GPU's (checked here)
+    rsgd, _, _ = ref.inisgdm(h["x"])
+    sgd_err = float(np.abs(rsgd - hs).max() / np.abs(rsgd).max())
     ref.stage_load(h["um"], h["u"], h["temp"], h["pres"], hj, hs, c4)
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except Exception:
         pass
     log("bench: timing the reference Fortran stage on one core ...")
+    sys.stdout.flush()
     devnull = os.open(os.devnull, os.O_WRONLY)
     saved = os.dup(1)
     os.dup2(devnull, 1)  # the reference prints per call
@@ -187,6 +193,7 @@ def cpu_baseline(f, jbn, sgd, c4, work, n, scheme):
         t0 = time.perf_counter()
         ref.stage_run()
         t = time.perf_counter() - t0
+        ref.flush()
     finally:
         os.dup2(saved, 1)
         os.close(devnull)
@@ -199,6 +206,7 @@ def cpu_baseline(f, jbn, sgd, c4, work, n, scheme):
 
     g = {k: v.cpu().numpy() for k, v in work.items()}
     parity = {"vs": "reference Fortran stage (same inputs, scheme %s)" % lib_scheme,
+              "sgd_setup": sgd_err,
               "dvel": rel(g["dvel"], dvel), "amut": rel(g["amut"], amut),
               "vis": rel(g["vis"], vis)}
     for m in range(5):

@@ -205,12 +205,16 @@ int emx_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
     set_error("emx_sgs_geometry: need at least 2 points per direction");
     return EMX_ERR_SIZE;
   }
-  const int nb = 1024;
-  double* bm = ctx->sgs_tmp;  // nb partial maxima
+  // one partial maximum per block, in the 4N-double scratch: nb <= N/256+1
+  // blocks always fit
+  const long long N = ctx->d.N();
+  const int nb = (int)std::min<long long>(1024, (N + 255) / 256);
+  double* bm = ctx->sgs_tmp;
   EMX_TRY(op_sgs_geometry(ctx, x, sgd, dmins, bm, nb));
   if (deltamax) {
-    double h[nb];
-    EMX_TRY_HIP(hipMemcpyAsync(h, bm, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    double h[1024];
+    EMX_TRY_HIP(hipMemcpyAsync(h, bm, nb * sizeof(double), hipMemcpyDeviceToHost,
+                               ctx->stream));
     EMX_TRY_HIP(hipStreamSynchronize(ctx->stream));
     double m = 0.0;
     for (int b = 0; b < nb; b++) m = h[b] > m ? h[b] : m;

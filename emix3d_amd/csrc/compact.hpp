@@ -87,18 +87,43 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
   const bool act = lane < T.P;
 
   double w[NV][W];
+  if constexpr (C >= 3) {
+    // own points from memory, the 3-point halos from the neighbour lanes
 #pragma unroll
-  for (int q = 0; q < W; q++) {
-    const int l = clampi(s - 3 + q, 0, n - 1);
-    double v[NV];
-    if (act) {
-      prob.load(base + (long long)l * sl, v);
-    } else {
+    for (int q = 0; q < C; q++) {
+      const int l = s + q;
+      double v[NV];
+      if (act && l < n) {
+        prob.load(base + (long long)l * sl, v);
+      } else {
 #pragma unroll
-      for (int a = 0; a < NV; a++) v[a] = 0.0;
+        for (int a = 0; a < NV; a++) v[a] = 0.0;
+      }
+#pragma unroll
+      for (int a = 0; a < NV; a++) w[a][q + 3] = v[a];
     }
 #pragma unroll
-    for (int a = 0; a < NV; a++) w[a][q] = v[a];
+    for (int a = 0; a < NV; a++) {
+#pragma unroll
+      for (int h = 0; h < 3; h++) {
+        w[a][h] = __shfl_up(w[a][C + h], 1, 64);
+        w[a][C + 3 + h] = __shfl_down(w[a][3 + h], 1, 64);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+      const int l = clampi(s - 3 + q, 0, n - 1);
+      double v[NV];
+      if (act) {
+        prob.load(base + (long long)l * sl, v);
+      } else {
+#pragma unroll
+        for (int a = 0; a < NV; a++) v[a] = 0.0;
+      }
+#pragma unroll
+      for (int a = 0; a < NV; a++) w[a][q] = v[a];
+    }
   }
 
   // local forward sweep from a zero carry

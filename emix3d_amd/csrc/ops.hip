@@ -368,27 +368,37 @@ int op_deriv(emx_ctx* ctx, int dir, const double* f, double* df, double ih) {
   return launch_lines(ctx, dir, p, ih);
 }
 
+// Derivatives of m independent fields along dir, two fields per launch (a
+// line solve holds 2*(2C+6) doubles per field in registers; pairs keep the
+// strided kernel within the 128 VGPRs its 1024-thread workgroups allow).
+static int deriv_fields(emx_ctx* ctx, int dir, int m, const double* const* in,
+                        double* const* out, double ih) {
+  int f = 0;
+  for (; f + 1 < m; f += 2) {
+    ProbDeriv<2> p;
+    p.in[0] = in[f];
+    p.in[1] = in[f + 1];
+    p.out[0] = out[f];
+    p.out[1] = out[f + 1];
+    EMX_TRY(launch_lines(ctx, dir, p, ih));
+  }
+  if (f < m) {
+    ProbDeriv<1> p;
+    p.in[0] = in[f];
+    p.out[0] = out[f];
+    EMX_TRY(launch_lines(ctx, dir, p, ih));
+  }
+  return EMX_OK;
+}
+
 int op_grad(emx_ctx* ctx, const double* u, const double* temp, double* dvel,
             double* dtemp, const double ih[3]) {
   const long long N = ctx->d.N();
   for (int dir = 0; dir < 3; dir++) {
-    if (temp) {
-      ProbDeriv<4> p;
-      for (int c = 0; c < 3; c++) {
-        p.in[c] = u + c * N;
-        p.out[c] = dvel + (3 * dir + c) * N;
-      }
-      p.in[3] = temp;
-      p.out[3] = dtemp + dir * N;
-      EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
-    } else {
-      ProbDeriv<3> p;
-      for (int c = 0; c < 3; c++) {
-        p.in[c] = u + c * N;
-        p.out[c] = dvel + (3 * dir + c) * N;
-      }
-      EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
-    }
+    const double* in[4] = {u, u + N, u + 2 * N, temp};
+    double* out[4] = {dvel + (3 * dir) * N, dvel + (3 * dir + 1) * N,
+                      dvel + (3 * dir + 2) * N, temp ? dtemp + dir * N : nullptr};
+    EMX_TRY(deriv_fields(ctx, dir, temp ? 4 : 3, in, out, ih[dir]));
   }
   return EMX_OK;
 }
@@ -448,12 +458,9 @@ int op_metrics(emx_ctx* ctx, const double* x, double* jbn, int* bad_dev) {
                         (double)(ctx->d.nz - 1)};
   // jacob_malla.f90:14-122: comp(x_c along dir d) = 1 + 3*c + d
   for (int dir = 0; dir < 3; dir++) {
-    ProbDeriv<3> p;
-    for (int c = 0; c < 3; c++) {
-      p.in[c] = x + c * N;
-      p.out[c] = jbn + (3 * c + dir) * N;
-    }
-    EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
+    const double* in[3] = {x, x + N, x + 2 * N};
+    double* out[3] = {jbn + dir * N, jbn + (3 + dir) * N, jbn + (6 + dir) * N};
+    EMX_TRY(deriv_fields(ctx, dir, 3, in, out, ih[dir]));
   }
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, ctx->stream, bad_dev, 0);
   hipLaunchKernelGGL(k_metric_invert, dim3(pointwise_blocks(N)), dim3(256), 0,

@@ -47,3 +47,31 @@ for scheme in c4 c6f; do
   $FC -shared $FFLAGS "${OBJS[@]}" -o "$OUT/libemx_ref_$scheme.so"
 done
 echo "build_ref: built $OUT/libemx_ref_c4.so $OUT/libemx_ref_c6f.so"
+
+# Drop-in check program: the reference's own GPU-facing Fortran wrappers
+# (deriv_vel_cuda.f90, divergencia_cuda.f90, advectLS.f90), unmodified, with
+# emix3d_amd/fortran/emix3d_shim.f90 and oracle/dropin_driver.f90, linked
+# against libemix3d_hip.so instead of the reference's CUDA objects.
+LIBDIR="$(cd "$HERE/.." && pwd)/emix3d_amd"
+if [ -f "$LIBDIR/libemix3d_hip.so" ]; then
+  B="$OUT/build_dropin"
+  mkdir -p "$B"
+  DSRCS=(
+    "$REF/src/MSVSingle/flow_mod_incomp2.f90"
+    "$REF/solver/multiphase_mod.f90"
+    "$REF/solver/deriv_vel_cuda.f90"
+    "$REF/solver/divergencia_cuda.f90"
+    "$REF/solver/advectLS.f90"
+    "$LIBDIR/fortran/emix3d_shim.f90"
+    "$HERE/dropin_driver.f90"
+  )
+  DOBJS=()
+  for s in "${DSRCS[@]}"; do
+    o="$B/$(basename "${s%.f90}").o"
+    (cd "$B" && $FC $FFLAGS -I"$REF/include" -c "$s" -o "$o")
+    DOBJS+=("$o")
+  done
+  $FC $FFLAGS "${DOBJS[@]}" -L"$LIBDIR" -lemix3d_hip \
+      -Wl,-rpath,'$ORIGIN/../../emix3d_amd' -o "$OUT/emx_dropin"
+  echo "build_ref: built $OUT/emx_dropin"
+fi

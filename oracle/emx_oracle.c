@@ -728,6 +728,23 @@ static void ls_advect_rhs(int Nx, int Ny, int Nz, double *rs,
     }
 }
 
+/* The two one-sided WENO5 derivatives of phi on a padded grid (the
+ * PhiDevPlus/MinusParameterJB pair after Dev1thO_Downwind), 3 comps each. */
+int orc_weno5_derivs(int Nx, int Ny, int Nz, const double *phi,
+                     const double *jbn, double dx, double dy, double dz,
+                     double *left, double *right)
+{
+    if (Nx < 4 || Ny < 4 || Nz < 4) return -1;
+    long NG = (long)Nx * Ny * Nz;
+    double *dphi = (double *)malloc(sizeof(double) * 3 * (size_t)NG);
+    if (!dphi) return -3;
+    ls_backward_diff(Nx, Ny, Nz, dphi, phi, 1.0 / dx, 1.0 / dy, 1.0 / dz);
+    ls_weno_left(Nx, Ny, Nz, left, dphi, jbn);
+    ls_weno_right(Nx, Ny, Nz, right, dphi, jbn);
+    free(dphi);
+    return 0;
+}
+
 int orc_ls_advect_weno5_padded(int Nx, int Ny, int Nz, double *phi,
                                const double *vel, const double *jbn,
                                double dx, double dy, double dz, double dt)

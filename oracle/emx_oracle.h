@@ -99,6 +99,12 @@ int orc_ls_advect_weno5_padded(int ngx, int ngy, int ngz, double *phi,
                                const double *vel, const double *jbn,
                                double dx, double dy, double dz, double dt);
 
+/* The left- and right-biased WENO5 derivatives (3 comps each) that one
+ * stage of orc_ls_advect_weno5_padded forms from phi. */
+int orc_weno5_derivs(int ngx, int ngy, int ngz, const double *phi,
+                     const double *jbn, double dx, double dy, double dz,
+                     double *left, double *right);
+
 /* Same step on unpadded arrays: mirror-pad by 5 cells the way advect_CUDA
  * pads its inputs (lsTransportEc.cu:100-166), step, copy the interior back. */
 int orc_ls_advect_weno5(int nx, int ny, int nz, double *phi, const double *vel,

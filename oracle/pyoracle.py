@@ -59,6 +59,7 @@ def lib():
             "orc_ls_advect_weno5_padded": [_I, _I, _I, _P, _P, _P, _D, _D, _D, _D],
             "orc_ls_advect_weno5": [_I, _I, _I, _P, _P, _P, _D, _D, _D, _D],
             "orc_pad_mirror": [_I, _I, _I, _I, _P, _P],
+            "orc_weno5_derivs": [_I, _I, _I, _P, _P, _D, _D, _D, _P, _P],
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
         }
         for name, args in sig.items():
@@ -174,6 +175,16 @@ def ls_advect_weno5_padded(phi, vel, jbn, dxyz, dt):
     return out
 
 
+def weno5_derivs(phi, jbn, dxyz):
+    """Left/right-biased WENO5 derivatives of phi on a padded grid."""
+    nx, ny, nz = dims(phi)
+    left = np.empty((3, nz, ny, nx))
+    right = np.empty((3, nz, ny, nx))
+    _check(lib().orc_weno5_derivs(nx, ny, nz, _ptr(phi), _ptr(jbn), dxyz[0], dxyz[1], dxyz[2],
+                                  _ptr(left), _ptr(right)), "weno5_derivs")
+    return left, right
+
+
 def pad_mirror(a, g=5):
     nx, ny, nz = dims(a)
     out = np.empty((nz + 2 * g, ny + 2 * g, nx + 2 * g))
@@ -276,6 +287,10 @@ class RefFortran:
 
     def stage_run(self):
         self.L.ref_stage_run()
+
+    def flush(self):
+        """Flush the Fortran runtime's buffered standard output."""
+        self.L.ref_flush()
 
     def stage_fetch(self):
         shp = self._keep[1].shape[1:]

@@ -354,3 +354,9 @@ subroutine ref_stage_fetch(rsout, rsvout, amutout, visout, dvout) &
   visout = vis
   dvout = dvel
 end subroutine ref_stage_fetch
+
+! Flush the Fortran runtime's buffered standard output (the reference prints
+! from divergencia and viscosidad_t), so callers can redirect it.
+subroutine ref_flush() bind(C, name='ref_flush')
+  flush(6)
+end subroutine ref_flush
