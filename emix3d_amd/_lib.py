@@ -23,8 +23,8 @@ SIGNATURES = {
     "emx_create": ([ctypes.POINTER(_P), _I, _I, _I, _I, _P], _I),
     "emx_destroy": ([_P], _I),
     "emx_set_stream": ([_P, _P], _I),
-    "emx_set_plan": ([_P, _I, _I, _I], _I),
-    "emx_get_plan": ([_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "emx_set_plan": ([_P, _I, _I, _I, _I], _I),
+    "emx_get_plan": ([_P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "emx_sync": ([_P], _I),
     "emx_deriv": ([_P, _I, _P, _P, _D], _I),
     "emx_grad": ([_P, _P, _P, _P, _P, _D, _D, _D], _I),
@@ -34,6 +34,7 @@ SIGNATURES = {
     "emx_flux": ([_P, _I, _P, _P, _P, _P, _P, _P, _P], _I),
     "emx_divergence": ([_P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div": ([_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
+    "emx_flux_div_all": ([_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_ls_extrapolate": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
     "emx_ls_advect_weno5": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
     "cudaDeriv": ([_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _I, _I, _I], None),

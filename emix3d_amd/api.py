@@ -17,7 +17,8 @@ import torch
 from . import _lib
 
 SCHEMES = {"c4": 0, "c6f": 1, "c6c": 2}
-KIND_WAVE, KIND_STRIDED = 0, 1
+KIND_WAVE, KIND_STRIDED, KIND_STRIDED32, KIND_STRIDED16 = 0, 1, 2, 3
+PLAN_DERIV, PLAN_FLUX = 0, 1
 
 
 def _ptr(t):
@@ -81,12 +82,14 @@ class Operator:
         nx, ny, nz = self.shape
         return (float(nx - 1), float(ny - 1), float(nz - 1))
 
-    def set_plan(self, direction, kind, chunk):
-        _lib.call("emx_set_plan", self._ctx, direction, kind, chunk)
+    def set_plan(self, direction, kind, chunk, op=PLAN_DERIV):
+        """Line-solver shape of an operation class (PLAN_DERIV: derivatives,
+        gradient, metrics; PLAN_FLUX: flux divergence) along a direction."""
+        _lib.call("emx_set_plan", self._ctx, op, direction, kind, chunk)
 
-    def get_plan(self, direction):
+    def get_plan(self, direction, op=PLAN_DERIV):
         k, c = ctypes.c_int(), ctypes.c_int()
-        _lib.call("emx_get_plan", self._ctx, direction, ctypes.byref(k), ctypes.byref(c))
+        _lib.call("emx_get_plan", self._ctx, op, direction, ctypes.byref(k), ctypes.byref(c))
         return k.value, c.value
 
     # -------------------------------------------------------------- operators
@@ -173,6 +176,20 @@ class Operator:
                   self._check(jbn, 11, "jbn"), _ptr(dvel), *ih, _ptr(rs), _ptr(rsv))
         return rs, rsv
 
+    def flujos_divergencia_all(self, um, u, pres, vis, jbn, dvel, ih=None, rs=None,
+                               rsv=None):
+        """flujos(m) + divergencia(m) for m = 1..5 (the RUNGEK equation loop)
+        in three directional passes; rs, rsv: 5 components."""
+        ih = self.default_ih() if ih is None else ih
+        rs = self._empty(5) if rs is None else rs
+        rsv = self._empty(5) if rsv is None else rsv
+        self._stream()
+        _lib.call("emx_flux_div_all", self._ctx, self._check(um, 5, "um"),
+                  self._check(u, 3, "u"), self._check(pres), self._check(vis),
+                  self._check(jbn, 11, "jbn"), self._check(dvel, 9, "dvel"), *ih,
+                  self._check(rs, 5, "rs"), self._check(rsv, 5, "rsv"))
+        return rs, rsv
+
     def advect_level_set(self, pres, jbn, sobject, dxyz, deltamin):
         """advectLevelSet: extrapolate pres into the solid (in place)."""
         self._stream()
@@ -196,9 +213,8 @@ class Operator:
         self.deriv_vel(u, temp, ih, dvel=w["dvel"], dtemp=w["dtemp"])
         self.viscosidad_t(u, w["dvel"], jbn, sgd, c4, amut=w["amut"], vis=w["vis"],
                           want_vis5=False)
-        for m in range(1, 6):
-            self.flujos_divergencia(m, um, u, pres, w["vis"], jbn, w["dvel"], ih,
-                                    rs=w["rs"][m - 1], rsv=w["rsv"][m - 1])
+        self.flujos_divergencia_all(um, u, pres, w["vis"], jbn, w["dvel"], ih,
+                                    rs=w["rs"], rsv=w["rsv"])
         return w
 
     def stage_buffers(self):

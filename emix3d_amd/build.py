@@ -12,7 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemix3d_hip.so")
 
-SOURCES = ["lines.hip", "ops.hip", "levelset.hip", "capi.hip"]
+SOURCES = ["lines.hip", "ops_deriv.hip", "ops_flux.hip", "ops_sgs.hip", "levelset.hip",
+           "capi.hip"]
 
 # -ffp-contract=off keeps the FP64 operation order of the reference (no
 # fused multiply-adds), so GPU results sit within a few ulps of the oracle.

@@ -26,17 +26,18 @@ int check_ctx(const emx_ctx* ctx) {
     }                                           \
   } while (0)
 
-int make_plan(emx_ctx* ctx, int dir, int kind, int C) {
+int make_plan(emx_ctx* ctx, int op, int dir, int kind, int C) {
   const int n = dir == 0 ? ctx->d.nx : (dir == 1 ? ctx->d.ny : ctx->d.nz);
-  if (!valid_chunk(kind, C)) {
-    set_error("invalid line plan kind %d C %d", kind, C);
+  if (!valid_plan(op, kind, C)) {
+    set_error("line plan kind %d C %d is not available for operation class %d",
+              kind, C, op);
     return EMX_ERR_ARG;
   }
   LineTab t{};
   EMX_TRY(get_line_tab(ctx->scheme, n, C, kind, &t));
-  ctx->plan[dir].tab = t;
-  ctx->plan[dir].kind = kind;
-  ctx->plan[dir].C = C;
+  ctx->plan[op][dir].tab = t;
+  ctx->plan[op][dir].kind = kind;
+  ctx->plan[op][dir].C = C;
   return EMX_OK;
 }
 
@@ -88,19 +89,20 @@ int emx_create(emx_ctx** out, int nx, int ny, int nz, int scheme,
     delete c;
     return EMX_ERR_HIP;
   }
-  for (int dir = 0; dir < 3; dir++) {
-    const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
-    int kind, C;
-    default_plan(dir, n, &kind, &C);
-    int rc = make_plan(c, dir, kind, C);
-    if (rc != EMX_OK) {
-      free_ctx(c);
-      return rc;
+  for (int op = 0; op < PLAN_OPS; op++)
+    for (int dir = 0; dir < 3; dir++) {
+      const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+      int kind, C;
+      default_plan(op, dir, n, &kind, &C);
+      int rc = make_plan(c, op, dir, kind, C);
+      if (rc != EMX_OK) {
+        free_ctx(c);
+        return rc;
+      }
     }
-  }
   const size_t N = (size_t)c->d.N();
   c->ls_bytes = ls_scratch_doubles(c->d) * sizeof(double);
-  if (hipMalloc(&c->acc, 2 * N * sizeof(double)) != hipSuccess ||
+  if (hipMalloc(&c->acc, 8 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->sgs_tmp, 4 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->ls_buf, c->ls_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -124,23 +126,23 @@ int emx_set_stream(emx_ctx* ctx, void* stream) {
   return EMX_OK;
 }
 
-int emx_set_plan(emx_ctx* ctx, int dir, int kind, int C) {
+int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C) {
   EMX_TRY(check_ctx(ctx));
-  if (dir < 0 || dir > 2) {
-    set_error("emx_set_plan: bad direction %d", dir);
+  if (dir < 0 || dir > 2 || op < 0 || op >= PLAN_OPS) {
+    set_error("emx_set_plan: bad operation class %d or direction %d", op, dir);
     return EMX_ERR_ARG;
   }
-  return make_plan(ctx, dir, kind, C);
+  return make_plan(ctx, op, dir, kind, C);
 }
 
-int emx_get_plan(const emx_ctx* ctx, int dir, int* kind, int* C) {
+int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C) {
   EMX_TRY(check_ctx(ctx));
-  if (dir < 0 || dir > 2 || !kind || !C) {
+  if (dir < 0 || dir > 2 || op < 0 || op >= PLAN_OPS || !kind || !C) {
     set_error("emx_get_plan: bad arguments");
     return EMX_ERR_ARG;
   }
-  *kind = ctx->plan[dir].kind;
-  *C = ctx->plan[dir].C;
+  *kind = ctx->plan[op][dir].kind;
+  *C = ctx->plan[op][dir].C;
   return EMX_OK;
 }
 
@@ -291,6 +293,23 @@ int emx_flux_div(emx_ctx* ctx, int neq, const double* um, const double* u,
   }
   const double ih[3] = {ihx, ihy, ihz};
   return op_flux_div(ctx, neq, FluxPtrs{um, u, pres, vis, jbn, dvel}, ih, rs, rsv);
+}
+
+int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
+                     const double* pres, const double* vis, const double* jbn,
+                     const double* dvel, double ihx, double ihy, double ihz,
+                     double* rs, double* rsv) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(um);
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  CHECK_PTR(vis);
+  CHECK_PTR(jbn);
+  CHECK_PTR(dvel);
+  CHECK_PTR(rs);
+  CHECK_PTR(rsv);
+  const double ih[3] = {ihx, ihy, ihz};
+  return op_flux_div_all(ctx, FluxPtrs{um, u, pres, vis, jbn, dvel}, ih, rs, rsv);
 }
 
 int emx_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,

@@ -69,64 +69,202 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
 }
 
 // ---------------------------------------------------------------------------
-// One wavefront per line.  Lines are numbered L = j + ny*k; the line's
-// first point is at L*line_base_stride... expressed through (nyl, sj, sk):
-// base(L) = (L % nyl)*sj + (L / nyl)*sk, point l at base + l*sl.
+// One wavefront per line (xi lines, contiguous in memory).  Lines are
+// numbered L = j + nyl*k; base(L) = (L % nyl)*sj + (L / nyl)*sk, point l at
+// base + l*sl.  Lane c owns the chunk [cC, cC+C).  Every global access is
+// coalesced (lane t touches points t + 64q); the NV values of a point are
+// formed once, then each field moves to the chunk order through a per-wave
+// LDS buffer and is solved on its own (few registers for many fields).
+// The Thomas factors are staged in LDS once per workgroup (4 lines).
 template <int S, int C, class Prob>
 __global__ void __launch_bounds__(256)
 k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
              long long sk, long long sl, double ih) {
   constexpr int NV = Prob::NV;
   constexpr int W = C + 6;
-  const int lane = threadIdx.x & 63;
-  const long long L = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (L >= nlines) return;  // wave-uniform: no block barrier follows
-  const long long base = (L % nyl) * sj + (L / nyl) * sk;
-  const int n = T.n;
-  const int s = lane * C;
-  const bool act = lane < T.P;
+  constexpr int STR = C + 1;  // chunk stride in the transpose buffer
+  extern __shared__ double lds[];
+  const int n = T.n, P = T.P;
+  const int ntab = 5 * n + 1;
+  double* s_ib = lds;
+  double* s_m = s_ib + n;
+  double* s_gam = s_m + n;
+  double* s_H = s_gam + n + 1;
+  double* s_G = s_H + n;
+  double* s_af = s_G + n;    // Af[384] Ab[384]
+  double* s_ab = s_af + 384;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double* xb = s_ab + 384 + wv * 64 * STR;
+  const long long L = (long long)blockIdx.x * 4 + wv;
+  const bool lact = L < nlines;
+  const long long Lc = lact ? L : nlines - 1;
+  const long long base = (Lc % nyl) * sj + (Lc / nyl) * sk;
 
-  double w[NV][W];
-  if constexpr (C >= 3) {
-    // own points from memory, the 3-point halos from the neighbour lanes
+  // coalesced loads (all issued before anything waits on them)
+  double v[C][NV];
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    const int idx = q * 64 + lane;
+    const int idc = idx < n ? idx : n - 1;
+    prob.load(base + (long long)idc * sl, v[q]);
+  }
+  for (int t = threadIdx.x; t < ntab + 768; t += 256)
+    lds[t] = T.blob[t < ntab ? t : t - ntab + (int)(T.Af - T.blob)];
+
+  const int s = lane * C;
+  const bool act = lane < P;
+#pragma unroll
+  for (int a = 0; a < NV; a++) {
+    // to the chunk order
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < C; q++) {
-      const int l = s + q;
-      double v[NV];
-      if (act && l < n) {
-        prob.load(base + (long long)l * sl, v);
-      } else {
-#pragma unroll
-        for (int a = 0; a < NV; a++) v[a] = 0.0;
-      }
-#pragma unroll
-      for (int a = 0; a < NV; a++) w[a][q + 3] = v[a];
+      const int idx = q * 64 + lane;
+      xb[idx + idx / C] = v[q][a];
     }
-#pragma unroll
-    for (int a = 0; a < NV; a++) {
-#pragma unroll
-      for (int h = 0; h < 3; h++) {
-        w[a][h] = __shfl_up(w[a][C + h], 1, 64);
-        w[a][C + 3 + h] = __shfl_down(w[a][3 + h], 1, 64);
-      }
-    }
-  } else {
+    __syncthreads();
+    double w[W];
 #pragma unroll
     for (int q = 0; q < W; q++) {
       const int l = clampi(s - 3 + q, 0, n - 1);
-      double v[NV];
-      if (act) {
-        prob.load(base + (long long)l * sl, v);
+      w[q] = xb[l + l / C];
+    }
+    // local forward sweep from a zero carry
+    double y[C];
+    double yp = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int l = s + q;
+      if (act && l < n) {
+        const double r = rhs_row<S>(&w[q + 3], l, n, ih);
+        yp = r * s_ib[l] - s_m[l] * yp;
+        y[q] = yp;
       } else {
-#pragma unroll
-        for (int a = 0; a < NV; a++) v[a] = 0.0;
+        y[q] = 0.0;
       }
+    }
+    // forward carries: inclusive weighted scan over lanes
+    {
+      double B = act ? yp : 0.0;
 #pragma unroll
-      for (int a = 0; a < NV; a++) w[a][q] = v[a];
+      for (int k = 0; k < 6; k++) {
+        const int d = 1 << k;
+        const double up = __shfl_up(B, d, 64);
+        if (lane >= d) B += s_af[k * 64 + lane] * up;
+      }
+      double Y = __shfl_up(B, 1, 64);
+      if (lane == 0) Y = 0.0;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (act && l < n) y[q] += s_H[l] * Y;
+      }
+    }
+    // local backward sweep from a zero carry
+    double xs = 0.0;
+#pragma unroll
+    for (int q = C - 1; q >= 0; q--) {
+      const int l = s + q;
+      if (act && l < n) {
+        xs = y[q] - s_gam[l + 1] * xs;
+        y[q] = xs;
+      }
+    }
+    // backward carries: weighted scan from the right, then the result
+    {
+      double B = act ? xs : 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        const int d = 1 << k;
+        const double dn = __shfl_down(B, d, 64);
+        if (lane + d < 64) B += s_ab[k * 64 + lane] * dn;
+      }
+      double X = __shfl_down(B, 1, 64);
+      if (lane == 63) X = 0.0;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (act && l < n) y[q] += s_G[l] * X;
+      }
+    }
+    // back to the coalesced order
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int l = s + q;
+      if (act && l < n) xb[l + l / C] = y[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int idx = q * 64 + lane;
+      v[q][a] = xb[idx + idx / C];
     }
   }
+  if (!lact) return;  // after the last barrier
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    const int idx = q * 64 + lane;
+    if (idx < n) prob.store(base + (long long)idx * sl, v[q]);
+  }
+}
 
-  // local forward sweep from a zero carry
+// ---------------------------------------------------------------------------
+// LG neighbouring lines per wavefront (lane = line within the group), the
+// 64/LG lane groups of a wave and the waves of the workgroup taking
+// consecutive chunks of C points along the line.  With LG = 64 every load
+// instruction of a wave reads 512 contiguous bytes; LG = 32 / 16 still read
+// whole 128-byte lines and let a line be cut into more, shorter chunks
+// (fewer registers per work-item).  Line of lane-in-group i and outer index o
+// starts at i*si + o*so; point l at +l*sl.  The Thomas factors and the banded
+// chunk-carry weights (weights below 1e-24 dropped) are staged in LDS while
+// the data loads are in flight; carries travel through LDS.
+template <int S, int C, int LG, class Prob>
+__global__ void __launch_bounds__(1024)
+k_lines_strided(Prob prob, LineTab T, int nlanes, long long si, long long so,
+                long long sl, double ih) {
+  constexpr int NV = Prob::NV;
+  constexpr int W = C + 6;
+  constexpr int PMAX = 1024 / LG;  // most chunks a workgroup holds
+  extern __shared__ double lds[];
+  const int n = T.n, P = T.P, K = T.K, KB = T.KB;
+  const int ntab = 5 * n + 1;
+  double* s_ib = lds;
+  double* s_m = s_ib + n;
+  double* s_gam = s_m + n;
+  double* s_H = s_gam + n + 1;
+  double* s_G = s_H + n;
+  double* s_wf = s_G + n;       // [P*KB]
+  double* s_wb = s_wf + P * KB;  // [P*KB]
+  double* sh = s_wb + P * KB;   // [NV][PMAX][LG]
+  const int tid = threadIdx.x;
+  const int li = tid % LG;
+  const int c = tid / LG;  // chunk index
+  const int i = blockIdx.x * LG + li;
+  const bool act = i < nlanes && c < P;
+  const int ic = i < nlanes ? i : nlanes - 1;
+  const int cc = c < P ? c : P - 1;
+  const long long base = (long long)ic * si + (long long)blockIdx.y * so;
+  const int s = c * C;
+
+  // window loads, unconditional (clamped addresses), all in flight at once
+  double w[NV][W];
+#pragma unroll
+  for (int q = 0; q < W; q++) {
+    const int l = clampi(cc * C - 3 + q, 0, n - 1);
+    double v[NV];
+    prob.load(base + (long long)l * sl, v);
+#pragma unroll
+    for (int a = 0; a < NV; a++) w[a][q] = v[a];
+  }
+  {
+    const int nstage = ntab + 2 * P * KB;
+    const int woff = (int)(T.WfB - T.blob);
+    for (int t = tid; t < nstage; t += blockDim.x)
+      lds[t] = T.blob[t < ntab ? t : t - ntab + woff];
+  }
+  __syncthreads();
+
   double y[NV][C];
   double yp[NV];
 #pragma unroll
@@ -135,7 +273,7 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
   for (int q = 0; q < C; q++) {
     const int l = s + q;
     if (act && l < n) {
-      const double ibl = T.ib[l], ml = T.m[l];
+      const double ibl = s_ib[l], ml = s_m[l];
 #pragma unroll
       for (int a = 0; a < NV; a++) {
         const double r = rhs_row<S>(&w[a][q + 3], l, n, ih);
@@ -147,25 +285,25 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
       for (int a = 0; a < NV; a++) y[a][q] = 0.0;
     }
   }
-  // forward carries: inclusive weighted scan over lanes
 #pragma unroll
-  for (int a = 0; a < NV; a++) {
-    double B = act ? yp[a] : 0.0;
+  for (int a = 0; a < NV; a++) sh[(a * PMAX + c) * LG + li] = act ? yp[a] : 0.0;
+  __syncthreads();
+  if (act) {
+    const int kk = c < K ? c : K;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      const int d = 1 << k;
-      const double up = __shfl_up(B, d, 64);
-      if (lane >= d) B += T.Af[k * 64 + lane] * up;
-    }
-    double Y = __shfl_up(B, 1, 64);
-    if (lane == 0) Y = 0.0;
+    for (int a = 0; a < NV; a++) {
+      double Y = 0.0;
+      for (int k = 0; k < kk; k++)
+        Y += s_wf[c * KB + k] * sh[(a * PMAX + c - 1 - k) * LG + li];
 #pragma unroll
-    for (int q = 0; q < C; q++) {
-      const int l = s + q;
-      if (act && l < n) y[a][q] += T.H[l] * Y;
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (l < n) y[a][q] += s_H[l] * Y;
+      }
     }
   }
-  // local backward sweep from a zero carry
+  __syncthreads();
+
   double xs[NV];
 #pragma unroll
   for (int a = 0; a < NV; a++) xs[a] = 0.0;
@@ -173,7 +311,7 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
   for (int q = C - 1; q >= 0; q--) {
     const int l = s + q;
     if (act && l < n) {
-      const double g1 = T.gam[l + 1];
+      const double g1 = s_gam[l + 1];
 #pragma unroll
       for (int a = 0; a < NV; a++) {
         xs[a] = y[a][q] - g1 * xs[a];
@@ -181,25 +319,24 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
       }
     }
   }
-  // backward carries: weighted scan from the right
+#pragma unroll
+  for (int a = 0; a < NV; a++) sh[(a * PMAX + c) * LG + li] = act ? xs[a] : 0.0;
+  __syncthreads();
+  if (!act) return;  // after the last barrier
+  const int kb = P - 1 - c < K ? P - 1 - c : K;
   double X[NV];
 #pragma unroll
   for (int a = 0; a < NV; a++) {
-    double B = act ? xs[a] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; k++) {
-      const int d = 1 << k;
-      const double dn = __shfl_down(B, d, 64);
-      if (lane + d < 64) B += T.Ab[k * 64 + lane] * dn;
-    }
-    X[a] = __shfl_down(B, 1, 64);
-    if (lane == 63) X[a] = 0.0;
+    double acc = 0.0;
+    for (int k = 0; k < kb; k++)
+      acc += s_wb[c * KB + k] * sh[(a * PMAX + c + 1 + k) * LG + li];
+    X[a] = acc;
   }
 #pragma unroll
   for (int q = 0; q < C; q++) {
     const int l = s + q;
-    if (act && l < n) {
-      const double Gl = T.G[l];
+    if (l < n) {
+      const double Gl = s_G[l];
       double v[NV];
 #pragma unroll
       for (int a = 0; a < NV; a++) v[a] = y[a][q] + Gl * X[a];
@@ -209,109 +346,116 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
 }
 
 // ---------------------------------------------------------------------------
-// 64 lines per workgroup (lane = line), P waves along the line.
-// Line of lane i and outer index o starts at i*si + o*so; point l at +l*sl.
-template <int S, int C, class Prob>
+// The strided shape for problems whose point values are expensive to form
+// (many input fields, e.g. the flux divergence): each work-item loads and
+// forms the NV values of its own C points only, then the fields are solved
+// one after another, the halo window of each read from an LDS tile
+// [n][LG] of the field.  Same launch geometry as k_lines_strided.
+template <int S, int C, int LG, class Prob>
 __global__ void __launch_bounds__(1024)
-k_lines_strided(Prob prob, LineTab T, int nlanes, long long si, long long so,
-                long long sl, double ih) {
+k_lines_tile(Prob prob, LineTab T, int nlanes, long long si, long long so,
+             long long sl, double ih) {
   constexpr int NV = Prob::NV;
   constexpr int W = C + 6;
-  __shared__ double sh[NV][16][64];
-  const int lane = threadIdx.x;
-  const int c = __builtin_amdgcn_readfirstlane(threadIdx.y);
-  const int n = T.n, P = T.P;
-  const int i = blockIdx.x * 64 + lane;
-  const bool act = i < nlanes;
-  const long long base = (long long)i * si + (long long)blockIdx.y * so;
+  constexpr int PMAX = 1024 / LG;
+  extern __shared__ double lds[];
+  const int n = T.n, P = T.P, K = T.K, KB = T.KB;
+  const int ntab = 5 * n + 1;
+  double* s_ib = lds;
+  double* s_m = s_ib + n;
+  double* s_gam = s_m + n;
+  double* s_H = s_gam + n + 1;
+  double* s_G = s_H + n;
+  double* s_wf = s_G + n;        // [P*KB]
+  double* s_wb = s_wf + P * KB;  // [P*KB]
+  double* sh = s_wb + P * KB;    // [PMAX][LG] carries
+  double* tile = sh + PMAX * LG; // [n][LG]
+  const int tid = threadIdx.x;
+  const int li = tid % LG;
+  const int c = tid / LG;
+  const int i = blockIdx.x * LG + li;
+  const bool act = i < nlanes && c < P;
+  const int ic = i < nlanes ? i : nlanes - 1;
+  const long long base = (long long)ic * si + (long long)blockIdx.y * so;
   const int s = c * C;
 
-  double w[NV][W];
-#pragma unroll
-  for (int q = 0; q < W; q++) {
-    const int l = clampi(s - 3 + q, 0, n - 1);
-    double v[NV];
-    if (act) {
-      prob.load(base + (long long)l * sl, v);
-    } else {
-#pragma unroll
-      for (int a = 0; a < NV; a++) v[a] = 0.0;
-    }
-#pragma unroll
-    for (int a = 0; a < NV; a++) w[a][q] = v[a];
-  }
-
-  double y[NV][C];
-  double yp[NV];
-#pragma unroll
-  for (int a = 0; a < NV; a++) yp[a] = 0.0;
+  double v[C][NV];
 #pragma unroll
   for (int q = 0; q < C; q++) {
-    const int l = s + q;
-    if (l < n) {
-      const double ibl = T.ib[l], ml = T.m[l];
-#pragma unroll
-      for (int a = 0; a < NV; a++) {
-        const double r = rhs_row<S>(&w[a][q + 3], l, n, ih);
-        yp[a] = r * ibl - ml * yp[a];
-        y[a][q] = yp[a];
-      }
-    } else {
-#pragma unroll
-      for (int a = 0; a < NV; a++) y[a][q] = 0.0;
-    }
+    const int l = s + q < n ? s + q : n - 1;
+    prob.load(base + (long long)l * sl, v[q]);
+  }
+  {
+    const int nstage = ntab + 2 * P * KB;
+    const int woff = (int)(T.WfB - T.blob);
+    for (int t = tid; t < nstage; t += blockDim.x)
+      lds[t] = T.blob[t < ntab ? t : t - ntab + woff];
   }
 #pragma unroll
-  for (int a = 0; a < NV; a++) sh[a][c][lane] = yp[a];
-  __syncthreads();
-#pragma unroll
   for (int a = 0; a < NV; a++) {
-    double Y = 0.0;
-    for (int t = 0; t < c; t++) Y += T.Wf[c * P + t] * sh[a][t][lane];
+    __syncthreads();
+    if (c < P) {
+#pragma unroll
+      for (int q = 0; q < C; q++)
+        if (s + q < n) tile[(s + q) * LG + li] = v[q][a];
+    }
+    __syncthreads();
+    double w[W];
+#pragma unroll
+    for (int q = 0; q < W; q++) w[q] = tile[clampi(s - 3 + q, 0, n - 1) * LG + li];
+    double y[C];
+    double yp = 0.0;
 #pragma unroll
     for (int q = 0; q < C; q++) {
       const int l = s + q;
-      if (l < n) y[a][q] += T.H[l] * Y;
-    }
-  }
-  __syncthreads();
-
-  double xs[NV];
-#pragma unroll
-  for (int a = 0; a < NV; a++) xs[a] = 0.0;
-#pragma unroll
-  for (int q = C - 1; q >= 0; q--) {
-    const int l = s + q;
-    if (l < n) {
-      const double g1 = T.gam[l + 1];
-#pragma unroll
-      for (int a = 0; a < NV; a++) {
-        xs[a] = y[a][q] - g1 * xs[a];
-        y[a][q] = xs[a];
+      if (act && l < n) {
+        const double r = rhs_row<S>(&w[q + 3], l, n, ih);
+        yp = r * s_ib[l] - s_m[l] * yp;
+        y[q] = yp;
+      } else {
+        y[q] = 0.0;
       }
     }
-  }
+    sh[c * LG + li] = act ? yp : 0.0;
+    __syncthreads();
+    if (act) {
+      const int kk = c < K ? c : K;
+      double Y = 0.0;
+      for (int k = 0; k < kk; k++) Y += s_wf[c * KB + k] * sh[(c - 1 - k) * LG + li];
 #pragma unroll
-  for (int a = 0; a < NV; a++) sh[a][c][lane] = xs[a];
-  __syncthreads();
-  double X[NV];
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (l < n) y[q] += s_H[l] * Y;
+      }
+    }
+    double xs = 0.0;
 #pragma unroll
-  for (int a = 0; a < NV; a++) {
-    double acc = 0.0;
-    for (int t = c + 1; t < P; t++) acc += T.Wb[c * P + t] * sh[a][t][lane];
-    X[a] = acc;
+    for (int q = C - 1; q >= 0; q--) {
+      const int l = s + q;
+      if (act && l < n) {
+        xs = y[q] - s_gam[l + 1] * xs;
+        y[q] = xs;
+      }
+    }
+    __syncthreads();
+    sh[c * LG + li] = act ? xs : 0.0;
+    __syncthreads();
+    if (act) {
+      const int kb = P - 1 - c < K ? P - 1 - c : K;
+      double X = 0.0;
+      for (int k = 0; k < kb; k++) X += s_wb[c * KB + k] * sh[(c + 1 + k) * LG + li];
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (l < n) v[q][a] = y[q] + s_G[l] * X;
+      }
+    }
   }
   if (!act) return;  // after the last barrier
 #pragma unroll
   for (int q = 0; q < C; q++) {
     const int l = s + q;
-    if (l < n) {
-      const double Gl = T.G[l];
-      double v[NV];
-#pragma unroll
-      for (int a = 0; a < NV; a++) v[a] = y[a][q] + Gl * X[a];
-      prob.store(base + (long long)l * sl, v);
-    }
+    if (l < n) prob.store(base + (long long)l * sl, v[q]);
   }
 }
 
@@ -351,48 +495,127 @@ template <int S, int C, class Prob>
 hipError_t launch_wave_c(const Prob& prob, const LineTab& T, const DirGeom& g,
                          double ih, hipStream_t st) {
   const long long blocks = (g.nlines + 3) / 4;
+  const size_t lds = (size_t)(5 * T.n + 1 + 768 + 4 * 64 * (C + 1)) * sizeof(double);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)k_lines_wave<S, C, Prob>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((k_lines_wave<S, C, Prob>), dim3((unsigned)blocks),
-                     dim3(256), 0, st, prob, T, g.nlines, g.nyl, g.sj, g.sk,
+                     dim3(256), lds, st, prob, T, g.nlines, g.nyl, g.sj, g.sk,
                      g.sl, ih);
   return hipGetLastError();
 }
 
-template <int S, int C, class Prob>
+template <class Prob, class = void>
+struct uses_tile { static constexpr bool value = false; };
+template <class Prob>
+struct uses_tile<Prob, decltype(void(Prob::TILE))> {
+  static constexpr bool value = Prob::TILE;
+};
+
+template <int S, int C, int LG, class Prob>
 hipError_t launch_strided_c(const Prob& prob, const LineTab& T,
                             const DirGeom& g, double ih, hipStream_t st) {
-  dim3 grid((g.nlanes + 63) / 64, g.nouter);
-  dim3 block(64, T.P);
-  hipLaunchKernelGGL((k_lines_strided<S, C, Prob>), grid, block, 0, st, prob,
-                     T, g.nlanes, g.si, g.so, g.sl, ih);
+  dim3 grid((g.nlanes + LG - 1) / LG, g.nouter);
+  const int threads = ((T.P * LG + 63) / 64) * 64;
+  if constexpr (uses_tile<Prob>::value) {
+    const size_t lds = (size_t)(5 * T.n + 1 + 2 * T.P * T.KB + (1024 / LG) * LG +
+                                T.n * LG) * sizeof(double);
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)k_lines_tile<S, C, LG, Prob>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_lines_tile<S, C, LG, Prob>), grid, dim3(threads), lds,
+                       st, prob, T, g.nlanes, g.si, g.so, g.sl, ih);
+    return hipGetLastError();
+  } else {
+  const size_t lds = (size_t)(5 * T.n + 1 + 2 * T.P * T.KB +
+                              Prob::NV * (1024 / LG) * LG) * sizeof(double);
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)k_lines_strided<S, C, LG, Prob>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_lines_strided<S, C, LG, Prob>), grid, dim3(threads), lds,
+                     st, prob, T, g.nlanes, g.si, g.so, g.sl, ih);
   return hipGetLastError();
+  }
 }
 
 template <int S, class Prob>
 hipError_t launch_lines_s(const Prob& prob, const LineTab& T, const DirGeom& g,
                           double ih, hipStream_t st) {
-  if (T.kind == KIND_WAVE) {
+  if constexpr (uses_tile<Prob>::value) {
+    // many-input problems: the wavefront kind, or the tile form of the
+    // 32/16-lines-per-wave kinds with short chunks (the formed values of a
+    // work-item's chunk must fit its registers)
+    switch (T.kind) {
+      case KIND_WAVE:
+        switch (T.C) {
+          case 1: return launch_wave_c<S, 1>(prob, T, g, ih, st);
+          case 2: return launch_wave_c<S, 2>(prob, T, g, ih, st);
+          case 4: return launch_wave_c<S, 4>(prob, T, g, ih, st);
+          case 8: return launch_wave_c<S, 8>(prob, T, g, ih, st);
+        }
+        break;
+      case KIND_STRIDED32:
+        if (T.C == 4) return launch_strided_c<S, 4, 32>(prob, T, g, ih, st);
+        if (T.C == 8) return launch_strided_c<S, 8, 32>(prob, T, g, ih, st);
+        break;
+      case KIND_STRIDED16:
+        if (T.C == 4) return launch_strided_c<S, 4, 16>(prob, T, g, ih, st);
+        if (T.C == 8) return launch_strided_c<S, 8, 16>(prob, T, g, ih, st);
+        break;
+    }
+    return hipErrorInvalidValue;
+  } else if constexpr (Prob::NV > 4) {
+    // more than four solves per work-item only fit the wavefront kind
+    if (T.kind != KIND_WAVE) return hipErrorInvalidValue;
     switch (T.C) {
       case 1: return launch_wave_c<S, 1>(prob, T, g, ih, st);
       case 2: return launch_wave_c<S, 2>(prob, T, g, ih, st);
       case 4: return launch_wave_c<S, 4>(prob, T, g, ih, st);
       case 8: return launch_wave_c<S, 8>(prob, T, g, ih, st);
-      case 16: return launch_wave_c<S, 16>(prob, T, g, ih, st);
     }
+    return hipErrorInvalidValue;
   } else {
-    switch (T.C) {
-      case 8: return launch_strided_c<S, 8>(prob, T, g, ih, st);
-      case 16: return launch_strided_c<S, 16>(prob, T, g, ih, st);
-      case 32: return launch_strided_c<S, 32>(prob, T, g, ih, st);
+    switch (T.kind) {
+      case KIND_WAVE:
+        switch (T.C) {
+          case 1: return launch_wave_c<S, 1>(prob, T, g, ih, st);
+          case 2: return launch_wave_c<S, 2>(prob, T, g, ih, st);
+          case 4: return launch_wave_c<S, 4>(prob, T, g, ih, st);
+          case 8: return launch_wave_c<S, 8>(prob, T, g, ih, st);
+          case 16: return launch_wave_c<S, 16>(prob, T, g, ih, st);
+        }
+        break;
+      case KIND_STRIDED:
+        switch (T.C) {
+          case 8: return launch_strided_c<S, 8, 64>(prob, T, g, ih, st);
+          case 16: return launch_strided_c<S, 16, 64>(prob, T, g, ih, st);
+          case 32: return launch_strided_c<S, 32, 64>(prob, T, g, ih, st);
+        }
+        break;
+      case KIND_STRIDED32:
+        switch (T.C) {
+          case 4: return launch_strided_c<S, 4, 32>(prob, T, g, ih, st);
+          case 8: return launch_strided_c<S, 8, 32>(prob, T, g, ih, st);
+          case 16: return launch_strided_c<S, 16, 32>(prob, T, g, ih, st);
+        }
+        break;
+      case KIND_STRIDED16:
+        switch (T.C) {
+          case 4: return launch_strided_c<S, 4, 16>(prob, T, g, ih, st);
+          case 8: return launch_strided_c<S, 8, 16>(prob, T, g, ih, st);
+        }
+        break;
     }
+    return hipErrorInvalidValue;
   }
-  return hipErrorInvalidValue;
 }
 
 // Solve all lines of direction dir with the context plan.
 template <class Prob>
 int launch_lines(emx_ctx* ctx, int dir, const Prob& prob, double ih) {
   const DirGeom g = dir_geom(ctx->d, dir);
-  const LineTab& T = ctx->plan[dir].tab;
+  const int op = uses_tile<Prob>::value ? PLAN_FLUX : PLAN_DERIV;
+  const LineTab& T = ctx->plan[op][dir].tab;
   hipError_t e;
   switch (ctx->scheme) {
     case EMX_SCHEME_C4: e = launch_lines_s<EMX_SCHEME_C4>(prob, T, g, ih, ctx->stream); break;
@@ -409,24 +632,62 @@ int launch_lines(emx_ctx* ctx, int dir, const Prob& prob, double ih) {
 
 // Valid chunk sizes per kind.
 inline bool valid_chunk(int kind, int C) {
-  if (kind == KIND_WAVE) return C == 1 || C == 2 || C == 4 || C == 8 || C == 16;
-  return C == 8 || C == 16 || C == 32;
+  switch (kind) {
+    case KIND_WAVE: return C == 1 || C == 2 || C == 4 || C == 8 || C == 16;
+    case KIND_STRIDED: return C == 8 || C == 16 || C == 32;
+    case KIND_STRIDED32: return C == 4 || C == 8 || C == 16;
+    case KIND_STRIDED16: return C == 4 || C == 8;
+  }
+  return false;
 }
 
-// Default plan: xi lines one wave per line with the smallest C that fits
-// 64 lanes; eta/zeta lines strided with the smallest C giving <= 16 waves.
-inline void default_plan(int dir, int n, int* kind, int* C) {
+// Most chunks a line may have under each kind (64 lanes of one wave, or
+// 1024 work-items of a workgroup divided by the lines per wave).
+inline int max_chunks(int kind) {
+  switch (kind) {
+    case KIND_WAVE: return 64;
+    case KIND_STRIDED: return 16;
+    case KIND_STRIDED32: return 32;
+    case KIND_STRIDED16: return 64;
+  }
+  return 0;
+}
+
+// Default plans.  Derivatives: xi lines one wave per line with the smallest
+// C that fits 64 lanes; eta/zeta lines 16 per wave, C=8 (lines up to 512
+// points), else 64 per wave with the smallest C giving <= 16 waves.  Flux
+// divergence: the same xi shape; eta/zeta 16 per wave with C=4 (the tile
+// form, lines up to 256 points), else C=8.
+inline void default_plan(int op, int dir, int n, int* kind, int* C) {
   if (dir == 0) {
     *kind = KIND_WAVE;
     int c = 1;
     while (c * 64 < n) c *= 2;
+    if (op == PLAN_FLUX && c > 8) {  // long xi lines: the tile form
+      *kind = KIND_STRIDED16;
+      c = 8;
+    }
     *C = c;
+  } else if (op == PLAN_FLUX) {
+    *kind = KIND_STRIDED16;
+    *C = n <= 256 ? 4 : 8;
+  } else if (n <= 512) {
+    *kind = KIND_STRIDED16;
+    *C = 8;
   } else {
     *kind = KIND_STRIDED;
     int c = 8;
     while (c * 16 < n && c < 32) c *= 2;
     *C = c;
   }
+}
+
+// Shapes each plan may take.
+inline bool valid_plan(int op, int kind, int C) {
+  if (!valid_chunk(kind, C)) return false;
+  if (op == PLAN_FLUX)
+    return kind == KIND_WAVE ? C <= 8 : (kind != KIND_STRIDED && C <= 8);
+  return true;
 }
 
 }  // namespace emx

@@ -38,7 +38,12 @@ void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
   } while (0)
 
 // Kinds of line solver launch (see compact.hpp).
-enum LineKind { KIND_WAVE = 0, KIND_STRIDED = 1 };
+enum LineKind {
+  KIND_WAVE = 0,       // one wavefront per line, lanes along the line
+  KIND_STRIDED = 1,    // 64 lines per wave, chunks over the waves
+  KIND_STRIDED32 = 2,  // 32 lines per wave, two chunk slots per wave
+  KIND_STRIDED16 = 3,  // 16 lines per wave, four chunk slots per wave
+};
 
 // Device tables of one (scheme, n, chunk) combination.  All arrays live in
 // one device allocation; pointers are device addresses.  With the Thomas
@@ -55,15 +60,24 @@ struct LineTab {
   int C;      // points per chunk
   int P;      // chunks per line
   int kind;   // LineKind
+  int K;      // chunk carries further than K chunks away are below 1e-24
+  int KB;     // max(K, 1): width of the banded carry weights
+  // all arrays below live in one device block (blob), laid out as
+  //   ib[n] m[n] gam[n+1] H[n] G[n] | Af[384] Ab[384] | WfB[P*KB] WbB[P*KB]
+  //   | Wf[P*P] Wb[P*P]
+  // so that a kernel stages [0, 5n+1) plus its kind's weights into LDS.
+  const double* blob;
   const double* ib;   // [n]    1/bet_l
   const double* m;    // [n]    a_l/bet_l
   const double* gam;  // [n+1]  gam_l (gam_0 = gam_n = 0)
   const double* H;    // [n]    prod_{t=s_c..l} (-m_t)
   const double* G;    // [n]    prod_{t=l+1..e_c} (-gam_t)
-  const double* Wf;   // [P*P]  strided kind: forward carry weights
-  const double* Wb;   // [P*P]  strided kind: backward carry weights
   const double* Af;   // [6*64] wave kind: forward scan multipliers
   const double* Ab;   // [6*64] wave kind: backward scan multipliers
+  const double* WfB;  // [P*KB] strided kinds: WfB[c*KB+k] = Wf[c][c-1-k]
+  const double* WbB;  // [P*KB] strided kinds: WbB[c*KB+k] = Wb[c][c+1+k]
+  const double* Wf;   // [P*P]  forward carry weights (full)
+  const double* Wb;   // [P*P]  backward carry weights (full)
 };
 
 // Builds (once per device) and returns the table.  Thread-safe.
@@ -74,6 +88,9 @@ int host_thomas(int scheme, int n, std::vector<double>& a,
 int scheme_min_n(int scheme);
 
 // Launch configuration of one direction of one context.
+// The line-solver shape is chosen per operation class and direction.
+enum PlanOp { PLAN_DERIV = 0, PLAN_FLUX = 1, PLAN_OPS = 2 };
+
 struct DirPlan {
   LineTab tab;
   int kind;
@@ -93,10 +110,10 @@ struct emx_ctx {
   int scheme = EMX_SCHEME_C6C;
   emx::Dims d{0, 0, 0};
   hipStream_t stream = nullptr;
-  emx::DirPlan plan[3];
-  // persistent scratch (device): two accumulators for the fused
+  emx::DirPlan plan[emx::PLAN_OPS][3];  // [operation class][direction]
+  // persistent scratch (device): eight accumulators for the fused
   // flux-divergence, four fields for the eddy-viscosity kernels.
-  double* acc = nullptr;       // 2*N
+  double* acc = nullptr;       // 8*N
   double* sgs_tmp = nullptr;   // 4*N
   // level-set scratch (allocated lazily on first level-set call)
   double* ls_buf = nullptr;

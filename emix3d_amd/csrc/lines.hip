@@ -76,12 +76,11 @@ int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
   std::vector<double> a, b, c;
   EMX_TRY(host_thomas(scheme, n, a, b, c));
   const int P = (n + C - 1) / C;
-  if (kind == KIND_WAVE && P > 64) {
-    set_error("line of %d points needs %d chunks of %d > 64 lanes", n, P, C);
-    return EMX_ERR_SIZE;
-  }
-  if (kind == KIND_STRIDED && P > 16) {
-    set_error("line of %d points needs %d chunks of %d > 16 waves", n, P, C);
+  const int pmax = kind == KIND_WAVE ? 64 : kind == KIND_STRIDED ? 16
+                 : kind == KIND_STRIDED32 ? 32 : 64;
+  if (P > pmax) {
+    set_error("line of %d points needs %d chunks of %d, more than the %d "
+              "this launch shape (kind %d) holds", n, P, C, pmax, kind);
     return EMX_ERR_SIZE;
   }
   std::vector<double> bet(n), gam(n + 1, 0.0), ib(n), m(n), H(n), G(n);
@@ -120,6 +119,13 @@ int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
       Wb[(size_t)ch * P + t] = w;
     }
   }
+  // carries from chunks further than K away weigh less than 1e-24
+  int K = 0;
+  for (int ch = 0; ch < P; ch++)
+    for (int t = 0; t < P; t++) {
+      const double w = t < ch ? Wf[(size_t)ch * P + t] : (t > ch ? Wb[(size_t)ch * P + t] : 0.0);
+      if (std::fabs(w) >= 1e-24) K = std::max(K, std::abs(ch - t));
+    }
   std::vector<double> Af(6 * 64, 0.0), Ab(6 * 64, 0.0);
   for (int k = 0; k < 6; k++) {
     int d = 1 << k;
@@ -136,48 +142,67 @@ int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
       }
     }
   }
-  // one device block: ib m gam H G Wf Wb Af Ab
-  size_t off[10];
+  // banded carry weights of the strided kinds: WfB[c*KB+k] = Wf[c][c-1-k],
+  // WbB[c*KB+k] = Wb[c][c+1+k], k < KB
+  const int KB = std::max(K, 1);
+  std::vector<double> WfB((size_t)P * KB, 0.0), WbB((size_t)P * KB, 0.0);
+  for (int ch = 0; ch < P; ch++)
+    for (int k = 0; k < KB; k++) {
+      if (ch - 1 - k >= 0) WfB[(size_t)ch * KB + k] = Wf[(size_t)ch * P + ch - 1 - k];
+      if (ch + 1 + k < P) WbB[(size_t)ch * KB + k] = Wb[(size_t)ch * P + ch + 1 + k];
+    }
+  // one device block, in the order the kernels stage it into LDS:
+  //   ib m gam(n+1) H G | Af Ab (wave kind) | WfB WbB (strided kinds) | Wf Wb
+  size_t off[12];
   off[0] = 0;
-  off[1] = off[0] + n;          // m
-  off[2] = off[1] + n;          // gam (n+1)
-  off[3] = off[2] + n + 1;      // H
-  off[4] = off[3] + n;          // G
-  off[5] = off[4] + n;          // Wf
-  off[6] = off[5] + (size_t)P * P;  // Wb
-  off[7] = off[6] + (size_t)P * P;  // Af
-  off[8] = off[7] + 6 * 64;         // Ab
-  off[9] = off[8] + 6 * 64;
-  std::vector<double> host(off[9]);
+  off[1] = off[0] + n;                 // m
+  off[2] = off[1] + n;                 // gam (n+1)
+  off[3] = off[2] + n + 1;             // H
+  off[4] = off[3] + n;                 // G
+  off[5] = off[4] + n;                 // Af
+  off[6] = off[5] + 6 * 64;            // Ab
+  off[7] = off[6] + 6 * 64;            // WfB
+  off[8] = off[7] + (size_t)P * KB;    // WbB
+  off[9] = off[8] + (size_t)P * KB;    // Wf
+  off[10] = off[9] + (size_t)P * P;    // Wb
+  off[11] = off[10] + (size_t)P * P;
+  std::vector<double> host(off[11]);
   memcpy(&host[off[0]], ib.data(), n * sizeof(double));
   memcpy(&host[off[1]], m.data(), n * sizeof(double));
   memcpy(&host[off[2]], gam.data(), (n + 1) * sizeof(double));
   memcpy(&host[off[3]], H.data(), n * sizeof(double));
   memcpy(&host[off[4]], G.data(), n * sizeof(double));
-  memcpy(&host[off[5]], Wf.data(), (size_t)P * P * sizeof(double));
-  memcpy(&host[off[6]], Wb.data(), (size_t)P * P * sizeof(double));
-  memcpy(&host[off[7]], Af.data(), 6 * 64 * sizeof(double));
-  memcpy(&host[off[8]], Ab.data(), 6 * 64 * sizeof(double));
+  memcpy(&host[off[5]], Af.data(), 6 * 64 * sizeof(double));
+  memcpy(&host[off[6]], Ab.data(), 6 * 64 * sizeof(double));
+  memcpy(&host[off[7]], WfB.data(), (size_t)P * KB * sizeof(double));
+  memcpy(&host[off[8]], WbB.data(), (size_t)P * KB * sizeof(double));
+  memcpy(&host[off[9]], Wf.data(), (size_t)P * P * sizeof(double));
+  memcpy(&host[off[10]], Wb.data(), (size_t)P * P * sizeof(double));
   double* dev = nullptr;
-  if (hipMalloc(&dev, off[9] * sizeof(double)) != hipSuccess) {
-    set_error("hipMalloc of a %zu-byte line table failed", off[9] * sizeof(double));
+  if (hipMalloc(&dev, off[11] * sizeof(double)) != hipSuccess) {
+    set_error("hipMalloc of a %zu-byte line table failed", off[11] * sizeof(double));
     return EMX_ERR_NOMEM;
   }
-  EMX_TRY_HIP(hipMemcpy(dev, host.data(), off[9] * sizeof(double),
+  EMX_TRY_HIP(hipMemcpy(dev, host.data(), off[11] * sizeof(double),
                         hipMemcpyHostToDevice));
   t->n = n;
   t->C = C;
   t->P = P;
   t->kind = kind;
+  t->K = K;
+  t->KB = KB;
+  t->blob = dev;
   t->ib = dev + off[0];
   t->m = dev + off[1];
   t->gam = dev + off[2];
   t->H = dev + off[3];
   t->G = dev + off[4];
-  t->Wf = dev + off[5];
-  t->Wb = dev + off[6];
-  t->Af = dev + off[7];
-  t->Ab = dev + off[8];
+  t->Af = dev + off[5];
+  t->Ab = dev + off[6];
+  t->WfB = dev + off[7];
+  t->WbB = dev + off[8];
+  t->Wf = dev + off[9];
+  t->Wb = dev + off[10];
   return EMX_OK;
 }
 

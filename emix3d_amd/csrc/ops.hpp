@@ -23,6 +23,8 @@ int op_divergence(emx_ctx* ctx, const double* flux, const double* jbn,
                   const double ih[3], double* rs, double* rsv);
 int op_flux_div(emx_ctx* ctx, int neq, const FluxPtrs& fp, const double ih[3],
                 double* rs, double* rsv);
+int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
+                    double* rs, double* rsv);
 int op_metrics(emx_ctx* ctx, const double* x, double* jbn, int* bad_dev);
 int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
                     double* dmins, double* blockmax, int nblocks);

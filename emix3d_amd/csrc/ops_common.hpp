@@ -1,0 +1,194 @@
+// Shared device definitions of the operator kernels: the line-solver
+// problems (what a solve loads and stores), the flux formulas of flujos and
+// small pointwise helpers.
+#pragma once
+#include <cmath>
+
+#include "compact.hpp"
+#include "ops.hpp"
+
+namespace emx {
+
+// ---------------------------------------------------------------------------
+// Problems for the line solver.
+
+// NV independent fields in, NV derivatives out.
+template <int N_>
+struct ProbDeriv {
+  static constexpr int NV = N_;
+  const double* in[N_];
+  double* out[N_];
+  __device__ __forceinline__ void load(long long p, double* v) const {
+#pragma unroll
+    for (int a = 0; a < N_; a++) v[a] = in[a][p];
+  }
+  __device__ __forceinline__ void store(long long p, const double* v) const {
+#pragma unroll
+    for (int a = 0; a < N_; a++) out[a][p] = v[a];
+  }
+};
+
+// Inputs of flujos (solver/flujos_compact.f90) addressed by linear index.
+struct FluxIn {
+  const double* um;    // 5 comps
+  const double* u;     // 3 comps
+  const double* pres;
+  const double* vis;
+  const double* jbn;   // 11 comps
+  const double* dvel;  // 9 comps
+  long long N;
+  __device__ __forceinline__ double UM(int m, long long p) const { return um[(m - 1) * N + p]; }
+  __device__ __forceinline__ double U(int m, long long p) const { return u[(m - 1) * N + p]; }
+  __device__ __forceinline__ double J(int c, long long p) const { return jbn[(c - 1) * N + p]; }
+  __device__ __forceinline__ double D(int c, long long p) const { return dvel[(c - 1) * N + p]; }
+};
+
+// Convective flux of equation neq along dir (0:e, 1:f, 2:g),
+// solver/flujos_compact.f90:21-185.
+__device__ __forceinline__ double conv_flux(const FluxIn& F, int neq, int dir,
+                                            long long p) {
+  switch (neq) {
+    case 1:
+      return -F.UM(2 + dir, p);
+    case 2:
+    case 3:
+    case 4: {
+      const double v = -F.UM(2 + dir, p) * F.U(neq - 1, p);
+      return (dir == neq - 2) ? v - F.pres[p] : v;
+    }
+    default:
+      return -F.U(1 + dir, p) * F.UM(5, p);
+  }
+}
+
+// Viscous flux of equation neq (2..4) along dir (0:ev, 1:fv, 2:gv),
+// solver/flujos_compact.f90:54-173.  The 3x3 table of (metric, dvel)
+// pairs is symmetric: entry (a,b) couples velocity a and direction b.
+__device__ __forceinline__ double visc_flux(const FluxIn& F, int neq, int dir,
+                                            long long p) {
+  const double mu = F.vis[p];
+  const int a = neq - 2;  // velocity component 0..2
+  if (a == dir) {
+    // 2*mu*metric_a * d(u_a)/d(xi_a): dvel index 1, 5, 9
+    const int mc = 1 + 4 * a;
+    return mu * (2.0 * (F.J(mc, p) * F.D(1 + 4 * a, p)));
+  }
+  // off-diagonal pairs, with the reference's operand order:
+  //  (0,1),(1,0): J1*D2 + J5*D4    (0,2),(2,0): J1*D3 + J9*D7
+  //  (1,2),(2,1): J9*D8 + J5*D6
+  const int lo = a < dir ? a : dir, hi = a < dir ? dir : a;
+  if (lo == 0 && hi == 1) return mu * (F.J(1, p) * F.D(2, p) + F.J(5, p) * F.D(4, p));
+  if (lo == 0 && hi == 2) return mu * (F.J(1, p) * F.D(3, p) + F.J(9, p) * F.D(7, p));
+  return mu * (F.J(9, p) * F.D(8, p) + F.J(5, p) * F.D(6, p));
+}
+
+// Flux fields already formed (cuDivergence / divergencia input).
+struct FluxArr {
+  const double* flux;  // 6 comps e,f,g,ev,fv,gv
+  long long N;
+  __device__ __forceinline__ double conv(int dir, long long p) const { return flux[dir * N + p]; }
+  __device__ __forceinline__ double visc(int dir, long long p) const { return flux[(3 + dir) * N + p]; }
+};
+
+// Fluxes formed on the fly from the primitive inputs.
+struct FluxLive {
+  FluxIn in;
+  int neq;
+  __device__ __forceinline__ double conv(int dir, long long p) const { return conv_flux(in, neq, dir, p); }
+  __device__ __forceinline__ double visc(int dir, long long p) const { return visc_flux(in, neq, dir, p); }
+};
+
+// Projection E = J*(xi_x*e) (solver/divergencia.f90:228-240) fused with
+// the three-pass accumulation rs = (1/J)*((E_xi + F_eta) + G_zeta)
+// (:349-351).  Pass 0 writes the accumulators, pass 1 adds, pass 2
+// finishes with 1/J.  VISC selects the viscous pair.
+template <class Src, bool VISC>
+struct ProbDiv {
+  static constexpr int NV = VISC ? 2 : 1;
+  Src src;
+  const double* jbn;
+  long long N;
+  int dir, pass;
+  double* acc;
+  double* accv;
+  double* rs;
+  double* rsv;
+  __device__ __forceinline__ void load(long long p, double* v) const {
+    const double det = jbn[9 * N + p];
+    const double met = jbn[(4 * dir) * N + p];  // comps 1, 5, 9
+    v[0] = det * (met * src.conv(dir, p));
+    if constexpr (VISC) v[1] = det * (met * src.visc(dir, p));
+  }
+  __device__ __forceinline__ void store(long long p, const double* v) const {
+    if (pass == 0) {
+      acc[p] = v[0];
+      if constexpr (VISC) accv[p] = v[1];
+    } else if (pass == 1) {
+      acc[p] = acc[p] + v[0];
+      if constexpr (VISC) accv[p] = accv[p] + v[1];
+    } else {
+      const double ij = jbn[10 * N + p];
+      rs[p] = ij * (acc[p] + v[0]);
+      if constexpr (VISC) rsv[p] = ij * (accv[p] + v[1]);
+    }
+  }
+};
+
+// All five equations of the RUNGEK loop `DO m=1,nd: flujos(m);
+// divergencia(m)` (solver/rungek_ico.f90:83-95) in three directional passes.
+// Eight solves per direction: the convective fluxes of equations 1..5 and
+// the viscous fluxes of 2..4 (those of 1 and 5 are zero).  G selects the
+// group a launch solves: 0 = convective 1..4, 1 = convective 5 and viscous
+// 2..4 (four solves each, for the strided kinds), 2 = all eight.
+// Accumulator slots: 0..4 convective, 5..7 viscous.
+template <int G>
+struct ProbDivAll {
+  static constexpr int NV = G == 2 ? 8 : 4;
+  static constexpr bool TILE = true;  // many inputs per point: form once
+  static constexpr int SLOT0 = G == 1 ? 4 : 0;
+  FluxIn in;
+  int dir, pass;
+  double* acc;  // 8 fields
+  double* rs;   // 5 fields
+  double* rsv;  // 5 fields
+  __device__ __forceinline__ double val(int slot, long long p) const {
+    return slot < 5 ? conv_flux(in, slot + 1, dir, p) : visc_flux(in, slot - 3, dir, p);
+  }
+  __device__ __forceinline__ void load(long long p, double* v) const {
+    const long long N = in.N;
+    const double det = in.jbn[9 * N + p];
+    const double met = in.jbn[(4 * dir) * N + p];
+#pragma unroll
+    for (int a = 0; a < NV; a++) v[a] = det * (met * val(SLOT0 + a, p));
+  }
+  __device__ __forceinline__ void store(long long p, const double* v) const {
+    const long long N = in.N;
+    if (pass == 0) {
+#pragma unroll
+      for (int a = 0; a < NV; a++) acc[(SLOT0 + a) * N + p] = v[a];
+    } else if (pass == 1) {
+#pragma unroll
+      for (int a = 0; a < NV; a++) {
+        const long long q = (SLOT0 + a) * N + p;
+        acc[q] = acc[q] + v[a];
+      }
+    } else {
+      const double ij = in.jbn[10 * N + p];
+#pragma unroll
+      for (int a = 0; a < NV; a++) {
+        const int slot = SLOT0 + a;
+        const double r = ij * (acc[slot * N + p] + v[a]);
+        if (slot < 5) rs[slot * N + p] = r;
+        else rsv[(slot - 4) * N + p] = r;
+      }
+    }
+  }
+};
+
+inline unsigned pointwise_blocks(long long n) {
+  long long b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+}  // namespace emx

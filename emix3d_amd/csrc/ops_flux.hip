@@ -1,0 +1,106 @@
+// Fluxes (flujos), flux divergence (divergencia) and their fused forms.
+#include "ops_common.hpp"
+
+namespace emx {
+
+__global__ void k_fill(double* a, double v, long long n) {
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x)
+    a[p] = v;
+}
+
+__global__ void k_flujos(FluxIn F, int neq, double* flux) {
+  const long long N = F.N;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long long)gridDim.x * blockDim.x) {
+    double e[6];
+#pragma unroll
+    for (int d = 0; d < 3; d++) e[d] = conv_flux(F, neq, d, p);
+    if (neq >= 2 && neq <= 4) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) e[3 + d] = visc_flux(F, neq, d, p);
+    } else {
+      e[3] = e[4] = e[5] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) flux[q * N + p] = e[q];
+  }
+}
+
+int op_flux(emx_ctx* ctx, int neq, const FluxPtrs& fp, double* flux) {
+  FluxIn F{fp.um, fp.u, fp.pres, fp.vis, fp.jbn, fp.dvel, ctx->d.N()};
+  hipLaunchKernelGGL(k_flujos, dim3(pointwise_blocks(F.N)), dim3(256), 0,
+                     ctx->stream, F, neq, flux);
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+template <class Src, bool VISC>
+static int div_passes(emx_ctx* ctx, const Src& src, const double* jbn,
+                      const double ih[3], double* rs, double* rsv) {
+  const long long N = ctx->d.N();
+  ProbDiv<Src, VISC> p;
+  p.src = src;
+  p.jbn = jbn;
+  p.N = N;
+  p.acc = ctx->acc;
+  p.accv = ctx->acc + N;
+  p.rs = rs;
+  p.rsv = rsv;
+  for (int dir = 0; dir < 3; dir++) {
+    p.dir = dir;
+    p.pass = dir;
+    EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
+  }
+  return EMX_OK;
+}
+
+int op_divergence(emx_ctx* ctx, const double* flux, const double* jbn,
+                  const double ih[3], double* rs, double* rsv) {
+  FluxArr src{flux, ctx->d.N()};
+  return div_passes<FluxArr, true>(ctx, src, jbn, ih, rs, rsv);
+}
+
+int op_flux_div(emx_ctx* ctx, int neq, const FluxPtrs& fp, const double ih[3],
+                double* rs, double* rsv) {
+  const long long N = ctx->d.N();
+  FluxLive src{{fp.um, fp.u, fp.pres, fp.vis, fp.jbn, fp.dvel, N}, neq};
+  if (neq >= 2 && neq <= 4)
+    return div_passes<FluxLive, true>(ctx, src, fp.jbn, ih, rs, rsv);
+  EMX_TRY((div_passes<FluxLive, false>(ctx, src, fp.jbn, ih, rs, rsv)));
+  // the viscous fluxes of the continuity and scalar equations are zero
+  // (flujos_compact.f90:32-40,187-214), so their divergence is zero
+  hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
+                     ctx->stream, rsv, 0.0, N);
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
+                    double* rs, double* rsv) {
+  const long long N = ctx->d.N();
+  const FluxIn in{fp.um, fp.u, fp.pres, fp.vis, fp.jbn, fp.dvel, N};
+  for (int dir = 0; dir < 3; dir++) {
+    // one launch of all eight solves on the wavefront kind with short
+    // chunks (8*C formed values per work-item), else two launches of four
+    const DirPlan& pl = ctx->plan[PLAN_FLUX][dir];
+    if (pl.kind == KIND_WAVE && pl.C <= 4) {
+      ProbDivAll<2> p{in, dir, dir, ctx->acc, rs, rsv};
+      EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
+    } else {
+      ProbDivAll<0> p0{in, dir, dir, ctx->acc, rs, rsv};
+      EMX_TRY(launch_lines(ctx, dir, p0, ih[dir]));
+      ProbDivAll<1> p1{in, dir, dir, ctx->acc, rs, rsv};
+      EMX_TRY(launch_lines(ctx, dir, p1, ih[dir]));
+    }
+  }
+  // zero viscous divergence of the continuity and scalar equations
+  hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
+                     ctx->stream, rsv, 0.0, N);
+  hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
+                     ctx->stream, rsv + 4 * N, 0.0, N);
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+}  // namespace emx

@@ -68,11 +68,16 @@ int emx_create(emx_ctx** ctx, int nx, int ny, int nz, int scheme,
                void* stream);
 int emx_destroy(emx_ctx* ctx);
 int emx_set_stream(emx_ctx* ctx, void* stream);
-/* Line-solver launch shape of direction dir (0=xi,1=eta,2=zeta):
- * kind 0 = one wavefront per line, kind 1 = 64 lines per workgroup with the
- * line split over waves; C = points per work-item.  Used by the autotuner. */
-int emx_set_plan(emx_ctx* ctx, int dir, int kind, int C);
-int emx_get_plan(const emx_ctx* ctx, int dir, int* kind, int* C);
+/* Line-solver launch shape of operation class op (0 = derivatives, gradient
+ * and metrics; 1 = flux divergence) along direction dir (0=xi,1=eta,2=zeta):
+ * kind 0 = one wavefront per line (lanes along the line); kinds 1, 2, 3 =
+ * 64, 32 or 16 neighbouring lines per wavefront with the line split into
+ * chunks over the lanes and waves of a workgroup; C = points per chunk.
+ * Valid for op 0: kind 0 C in {1,2,4,8,16}; kind 1 {8,16,32}; kind 2
+ * {4,8,16}; kind 3 {4,8}.  Op 1: kind 0 C <= 8, kinds 2 and 3 C in {4,8}.
+ * Used by the autotuner. */
+int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C);
+int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C);
 int emx_sync(emx_ctx* ctx);
 
 /* ---- device-pointer operator entry points ------------------------------- */
@@ -125,6 +130,15 @@ int emx_flux_div(emx_ctx* ctx, int neq, const double* um, const double* u,
                  const double* pres, const double* vis, const double* jbn,
                  const double* dvel, double ihx, double ihy, double ihz,
                  double* rs, double* rsv);
+
+/* The whole equation loop of a RUNGEK sub-step, `DO m=1,nd: flujos(m);
+ * divergencia(m)` (solver/rungek_ico.f90:83-95), in three directional
+ * passes: rs and rsv hold 5 components (one per equation).  rsv of the
+ * continuity and scalar equations is zero, as in the reference. */
+int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
+                     const double* pres, const double* vis, const double* jbn,
+                     const double* dvel, double ihx, double ihy, double ihz,
+                     double* rs, double* rsv);
 
 /* Pressure extrapolation into the solid (sobject > 0): advect_CUDA
  * (src/LevelSet/lsTransportEc.cu:24-197 -> interface.cu:341-436).

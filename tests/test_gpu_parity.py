@@ -57,10 +57,15 @@ def test_derivative_all_directions(oracle, scheme, shape):
         report(f"{scheme} {shape} dir{d}", normwise_rel(got, ref))
 
 
-@pytest.mark.parametrize("kind,chunks", [(0, [1, 2, 4, 8, 16]), (1, [8, 16, 32])])
-def test_derivative_every_plan(oracle, kind, chunks):
-    """Every line-solver shape the autotuner may pick gives the same result."""
-    nx, ny, nz = 96, 80, 72
+PLANS = [(0, [1, 2, 4, 8, 16], 64), (1, [8, 16, 32], 16), (2, [4, 8, 16], 32),
+         (3, [4, 8], 64)]
+
+
+@pytest.mark.parametrize("kind,chunks,pmax", PLANS)
+def test_derivative_every_plan(oracle, kind, chunks, pmax):
+    """Every line-solver shape the autotuner may pick gives the same result
+    (sizes not multiples of the chunk or of the lines per wave)."""
+    nx, ny, nz = 96, 81, 70
     rng = np.random.default_rng(3)
     f = rng.standard_normal((nz, ny, nx))
     op = Operator(nx, ny, nz, "c6c")
@@ -70,7 +75,7 @@ def test_derivative_every_plan(oracle, kind, chunks):
         ref = oracle.deriv3d(2, d, f, float(n - 1))
         for C in chunks:
             P = -(-n // C)
-            if (kind == 0 and P > 64) or (kind == 1 and P > 16):
+            if P > pmax:
                 continue
             op.set_plan(d, kind, C)
             got = host(op.derivative(fd, d))
@@ -171,6 +176,35 @@ def test_flujos_divergencia(oracle, scheme):
                                           D["dvel"])
         report(f"fused rs{neq}", normwise_rel(host(rs2), rrs))
         report(f"fused rsv{neq}", normwise_rel(host(rsv2), rrsv))
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+@pytest.mark.parametrize("plan", ["default", "strided", "wave_x_strided16"])
+def test_flux_div_all_equations(oracle, scheme, plan):
+    """The RUNGEK equation loop in three passes equals flujos+divergencia of
+    each equation, for every grouping the launch shapes select."""
+    shape = (26, 21, 19)
+    z = stage_inputs(shape, 33, scheme, oracle)
+    _, vis, _ = oracle.sgdm(z["u"], z["dvel"], z["jbn"], z["sgd"], z["c4"])
+    op = Operator(*shape, scheme)
+    if plan == "strided":
+        op.set_plan(0, 2, 8, op=1)
+        op.set_plan(1, 2, 4, op=1)
+        op.set_plan(2, 3, 8, op=1)
+    elif plan == "wave_x_strided16":
+        op.set_plan(0, 0, 2, op=1)
+        op.set_plan(1, 2, 8, op=1)
+        op.set_plan(2, 3, 4, op=1)
+    ih = op.default_ih()
+    D = {k: dev(z[k]) for k in ("um", "u", "pres", "jbn", "dvel")}
+    rs, rsv = op.flujos_divergencia_all(D["um"], D["u"], D["pres"], dev(vis), D["jbn"],
+                                        D["dvel"])
+    rs, rsv = host(rs), host(rsv)
+    for neq in range(1, 6):
+        flux = oracle.flujos(neq, z["um"], z["u"], z["pres"], vis, z["jbn"], z["dvel"])
+        rrs, rrsv = oracle.divergencia(SID[scheme], flux, z["jbn"], ih)
+        report(f"all rs{neq}", normwise_rel(rs[neq - 1], rrs))
+        report(f"all rsv{neq}", normwise_rel(rsv[neq - 1], rrsv))
 
 
 def sphere_solid(shape, seed):
