@@ -25,6 +25,10 @@ SIGNATURES = {
     "emx_set_stream": ([_P, _P], _I),
     "emx_set_plan": ([_P, _I, _I, _I, _I], _I),
     "emx_get_plan": ([_P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "emx_timing_begin": ([_P], _I),
+    "emx_timing_end": ([_P, ctypes.POINTER(_I)], _I),
+    "emx_timing_entry": ([_P, _I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_I),
+                          ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_D)], _I),
     "emx_sync": ([_P], _I),
     "emx_deriv": ([_P, _I, _P, _P, _D], _I),
     "emx_grad": ([_P, _P, _P, _P, _P, _D, _D, _D], _I),
@@ -35,6 +39,11 @@ SIGNATURES = {
     "emx_divergence": ([_P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div": ([_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div_all": ([_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
+    "emx_deriv_adj": ([_P, _I, _P, _P, _D], _I),
+    "emx_grad_adj": ([_P, _P, _P, _P, _P, _D, _D, _D], _I),
+    "emx_divergence_adj": ([_P, _P, _P, _P, _D, _D, _D, _P], _I),
+    "emx_flux_div_all_adj": ([_P, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P, _P, _P, _P],
+                             _I),
     "emx_ls_extrapolate": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
     "emx_ls_advect_weno5": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
     "cudaDeriv": ([_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _I, _I, _I], None),

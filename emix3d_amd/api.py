@@ -92,6 +92,28 @@ class Operator:
         _lib.call("emx_get_plan", self._ctx, op, direction, ctypes.byref(k), ctypes.byref(c))
         return k.value, c.value
 
+    # ---------------------------------------------------------------- timing
+    def timing_begin(self):
+        """Start recording every launch of this context with HIP events on
+        its stream (emx_timing_begin)."""
+        _lib.call("emx_timing_begin", self._ctx)
+
+    def timing_end(self):
+        """Stop recording; returns one dict per (kernel, direction), largest
+        total time first: label, dir (-1 for pointwise kernels), launches,
+        ms (summed), bytes (summed algorithmic bytes)."""
+        n = ctypes.c_int()
+        _lib.call("emx_timing_end", self._ctx, ctypes.byref(n))
+        out = []
+        for i in range(n.value):
+            lab, d, cnt = ctypes.c_char_p(), ctypes.c_int(), ctypes.c_int()
+            ms, by = ctypes.c_double(), ctypes.c_double()
+            _lib.call("emx_timing_entry", self._ctx, i, ctypes.byref(lab), ctypes.byref(d),
+                      ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(by))
+            out.append({"label": lab.value.decode(), "dir": d.value, "launches": cnt.value,
+                        "ms": ms.value, "bytes": by.value})
+        return out
+
     # -------------------------------------------------------------- operators
     def derivative(self, f, direction, inv_h=None, out=None):
         """Compact first derivative along direction 0/1/2 (derivnper per line)."""
@@ -189,6 +211,57 @@ class Operator:
                   self._check(jbn, 11, "jbn"), self._check(dvel, 9, "dvel"), *ih,
                   self._check(rs, 5, "rs"), self._check(rsv, 5, "rsv"))
         return rs, rsv
+
+    # ------------------------------------------------------------- adjoints
+    def derivative_adj(self, g, direction, inv_h=None, out=None):
+        """Transpose of derivative(): out = D^T g along direction."""
+        ih = self.default_ih()[direction] if inv_h is None else inv_h
+        out = self._empty(1)[0] if out is None else out
+        self._stream()
+        _lib.call("emx_deriv_adj", self._ctx, direction, self._check(g), self._check(out),
+                  float(ih))
+        return out
+
+    def deriv_vel_adj(self, dvel_bar, dtemp_bar=None, ih=None, u_bar=None, temp_bar=None):
+        """Transpose of deriv_vel(): (u_bar[3], temp_bar or None)."""
+        ih = self.default_ih() if ih is None else ih
+        u_bar = self._empty(3) if u_bar is None else u_bar
+        if dtemp_bar is not None and temp_bar is None:
+            temp_bar = self._empty(1)[0]
+        self._stream()
+        _lib.call("emx_grad_adj", self._ctx, self._check(dvel_bar, 9, "dvel_bar"),
+                  None if dtemp_bar is None else self._check(dtemp_bar, 3, "dtemp_bar"),
+                  self._check(u_bar, 3, "u_bar"),
+                  None if dtemp_bar is None else self._check(temp_bar), *ih)
+        return u_bar, (temp_bar if dtemp_bar is not None else None)
+
+    def divergencia_adj(self, rs_bar, rsv_bar, jbn, ih=None, flux_bar=None):
+        """Transpose of divergencia() with respect to its six flux fields."""
+        ih = self.default_ih() if ih is None else ih
+        flux_bar = self._empty(6) if flux_bar is None else flux_bar
+        self._stream()
+        _lib.call("emx_divergence_adj", self._ctx, self._check(rs_bar), self._check(rsv_bar),
+                  self._check(jbn, 11, "jbn"), *ih, self._check(flux_bar, 6, "flux_bar"))
+        return flux_bar
+
+    def flujos_divergencia_all_adj(self, rs_bar, rsv_bar, um, u, vis, jbn, dvel, ih=None,
+                                   out=None):
+        """Transpose of the linearisation of flujos_divergencia_all() about
+        (um, u, vis, dvel); returns dict um_bar(5), u_bar(3), pres_bar,
+        vis_bar, dvel_bar(9)."""
+        ih = self.default_ih() if ih is None else ih
+        if out is None:
+            out = {"um_bar": self._empty(5), "u_bar": self._empty(3),
+                   "pres_bar": self._empty(1)[0], "vis_bar": self._empty(1)[0],
+                   "dvel_bar": self._empty(9)}
+        self._stream()
+        _lib.call("emx_flux_div_all_adj", self._ctx, self._check(rs_bar, 5, "rs_bar"),
+                  self._check(rsv_bar, 5, "rsv_bar"), self._check(um, 5, "um"),
+                  self._check(u, 3, "u"), self._check(vis), self._check(jbn, 11, "jbn"),
+                  self._check(dvel, 9, "dvel"), *ih, self._check(out["um_bar"], 5, "um_bar"),
+                  self._check(out["u_bar"], 3, "u_bar"), self._check(out["pres_bar"]),
+                  self._check(out["vis_bar"]), self._check(out["dvel_bar"], 9, "dvel_bar"))
+        return out
 
     def advect_level_set(self, pres, jbn, sobject, dxyz, deltamin):
         """advectLevelSet: extrapolate pres into the solid (in place)."""

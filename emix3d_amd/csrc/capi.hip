@@ -1,4 +1,5 @@
 // extern "C" boundary of libemix3d_hip.so (declared in include/emix3d.h).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -47,6 +48,7 @@ void free_ctx(emx_ctx* c) {
   (void)hipFree(c->sgs_tmp);
   (void)hipFree(c->ls_buf);
   (void)hipFree(c->stage);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   delete c;
 }
 
@@ -100,6 +102,18 @@ int emx_create(emx_ctx** out, int nx, int ny, int nz, int scheme,
         return rc;
       }
     }
+  // tables of the transposed operator (adjoint entry points), for lines
+  // the transposed solve holds
+  for (int dir = 0; dir < 3; dir++) {
+    const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    if (n > ADJ_MAX_N) continue;
+    int rc = get_line_tab(scheme | TAB_TRANSPOSED, n, adj_chunk(n), KIND_STRIDED16,
+                          &c->adj[dir]);
+    if (rc != EMX_OK) {
+      free_ctx(c);
+      return rc;
+    }
+  }
   const size_t N = (size_t)c->d.N();
   c->ls_bytes = ls_scratch_doubles(c->d) * sizeof(double);
   if (hipMalloc(&c->acc, 8 * N * sizeof(double)) != hipSuccess ||
@@ -143,6 +157,64 @@ int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C) {
   }
   *kind = ctx->plan[op][dir].kind;
   *C = ctx->plan[op][dir].C;
+  return EMX_OK;
+}
+
+int emx_timing_begin(emx_ctx* ctx) {
+  EMX_TRY(check_ctx(ctx));
+  ctx->timed.clear();
+  ctx->timing_summary.clear();
+  ctx->timing = true;
+  return EMX_OK;
+}
+
+int emx_timing_end(emx_ctx* ctx, int* n_entries) {
+  EMX_TRY(check_ctx(ctx));
+  const bool was = ctx->timing;
+  ctx->timing = false;
+  ctx->timing_summary.clear();
+  if (!ctx->timed.empty()) {
+    EMX_TRY_HIP(hipEventSynchronize(ctx->ev_pool[2 * ctx->timed.size() - 1]));
+    for (size_t i = 0; i < ctx->timed.size(); i++) {
+      float ms = 0.f;
+      EMX_TRY_HIP(hipEventElapsedTime(&ms, ctx->ev_pool[2 * i], ctx->ev_pool[2 * i + 1]));
+      const emx::TimedLaunch& t = ctx->timed[i];
+      emx::TimingEntry* e = nullptr;
+      for (auto& x : ctx->timing_summary)
+        if (x.dir == t.dir && std::strcmp(x.label, t.label) == 0) e = &x;
+      if (!e) {
+        ctx->timing_summary.push_back(emx::TimingEntry{t.label, t.dir, 0, 0.0, 0.0});
+        e = &ctx->timing_summary.back();
+      }
+      e->launches++;
+      e->ms += ms;
+      e->bytes += t.bytes;
+    }
+    std::sort(ctx->timing_summary.begin(), ctx->timing_summary.end(),
+              [](const emx::TimingEntry& a, const emx::TimingEntry& b) { return a.ms > b.ms; });
+  }
+  ctx->timed.clear();
+  if (n_entries) *n_entries = (int)ctx->timing_summary.size();
+  if (!was) {
+    set_error("emx_timing_end: timing was not running (or an event failed)");
+    return EMX_ERR_ARG;
+  }
+  return EMX_OK;
+}
+
+int emx_timing_entry(const emx_ctx* ctx, int i, const char** label, int* dir,
+                     int* launches, double* total_ms, double* bytes) {
+  EMX_TRY(check_ctx(ctx));
+  if (i < 0 || i >= (int)ctx->timing_summary.size()) {
+    set_error("emx_timing_entry: index %d out of range", i);
+    return EMX_ERR_ARG;
+  }
+  const emx::TimingEntry& e = ctx->timing_summary[i];
+  if (label) *label = e.label;
+  if (dir) *dir = e.dir;
+  if (launches) *launches = e.launches;
+  if (total_ms) *total_ms = e.ms;
+  if (bytes) *bytes = e.bytes;
   return EMX_OK;
 }
 
@@ -310,6 +382,72 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
   CHECK_PTR(rsv);
   const double ih[3] = {ihx, ihy, ihz};
   return op_flux_div_all(ctx, FluxPtrs{um, u, pres, vis, jbn, dvel}, ih, rs, rsv);
+}
+
+int emx_deriv_adj(emx_ctx* ctx, int dir, const double* g, double* out,
+                  double inv_h) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(g);
+  CHECK_PTR(out);
+  if (dir < 0 || dir > 2) {
+    set_error("emx_deriv_adj: bad direction %d", dir);
+    return EMX_ERR_ARG;
+  }
+  if (g == out) {
+    set_error("emx_deriv_adj: output aliases input");
+    return EMX_ERR_ARG;
+  }
+  return op_deriv_adj(ctx, dir, g, out, inv_h);
+}
+
+int emx_grad_adj(emx_ctx* ctx, const double* dvel_bar, const double* dtemp_bar,
+                 double* u_bar, double* temp_bar, double ihx, double ihy,
+                 double ihz) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(dvel_bar);
+  CHECK_PTR(u_bar);
+  if ((dtemp_bar == nullptr) != (temp_bar == nullptr)) {
+    set_error("emx_grad_adj: dtemp_bar and temp_bar must both be given or both be NULL");
+    return EMX_ERR_ARG;
+  }
+  const double ih[3] = {ihx, ihy, ihz};
+  return op_grad_adj(ctx, dvel_bar, dtemp_bar, u_bar, temp_bar, ih);
+}
+
+int emx_divergence_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_bar,
+                       const double* jbn, double ihx, double ihy, double ihz,
+                       double* flux_bar) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(rs_bar);
+  CHECK_PTR(rsv_bar);
+  CHECK_PTR(jbn);
+  CHECK_PTR(flux_bar);
+  const double ih[3] = {ihx, ihy, ihz};
+  return op_divergence_adj(ctx, rs_bar, rsv_bar, jbn, ih, flux_bar);
+}
+
+int emx_flux_div_all_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_bar,
+                         const double* um, const double* u, const double* vis,
+                         const double* jbn, const double* dvel, double ihx,
+                         double ihy, double ihz, double* um_bar, double* u_bar,
+                         double* pres_bar, double* vis_bar, double* dvel_bar) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(rs_bar);
+  CHECK_PTR(rsv_bar);
+  CHECK_PTR(um);
+  CHECK_PTR(u);
+  CHECK_PTR(vis);
+  CHECK_PTR(jbn);
+  CHECK_PTR(dvel);
+  CHECK_PTR(um_bar);
+  CHECK_PTR(u_bar);
+  CHECK_PTR(pres_bar);
+  CHECK_PTR(vis_bar);
+  CHECK_PTR(dvel_bar);
+  const double ih[3] = {ihx, ihy, ihz};
+  FluxPtrs fp{um, u, nullptr, vis, jbn, dvel};
+  return op_flux_div_all_adj(ctx, rs_bar, rsv_bar, fp, ih, um_bar, u_bar, pres_bar,
+                             vis_bar, dvel_bar);
 }
 
 int emx_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,

@@ -29,9 +29,11 @@ namespace emx {
 //  C4 : solver/derivada.f90:104-109 (consrs: :83-89)
 //  C6F: src/MSVSingle/derivada_incomp.f90:178-185 (consrs: :81-86)
 //  C6C: src/Schemes/derivCompact.cu:90-140
+// Also evaluated on the host (with unit vectors) to tabulate B^T for the
+// transposed solve.
 template <int S>
-__device__ __forceinline__ double rhs_row(const double* w, int l, int n,
-                                          double ih) {
+__host__ __device__ __forceinline__ double rhs_row(const double* w, int l, int n,
+                                                   double ih) {
   if constexpr (S == EMX_SCHEME_C4) {
     const double ars = 3. / 4., ars1 = -17. / 6., brs1 = 3. / 2., crs1 = 1.5;
     if (l == 0)
@@ -460,6 +462,129 @@ k_lines_tile(Prob prob, LineTab T, int nlanes, long long si, long long so,
 }
 
 // ---------------------------------------------------------------------------
+// Transposed compact derivative along lines: out = (ih A^{-1} B)^T g
+// = ih B^T A^{-T} g.  T is a TAB_TRANSPOSED table: the Thomas factors and
+// chunk-carry weights of A^T, and B^T as seven diagonals.  Each work-item
+// solves its chunk of A^T y = g (carries as in k_lines_tile), the solution
+// goes to an LDS tile [n][LG] of the field, and B^T is applied from the
+// tile.  The NV fields of a point are solved one after another.  Launch
+// geometry of k_lines_tile.
+template <int C, int LG, class Prob>
+__global__ void __launch_bounds__(1024)
+k_lines_adj(Prob prob, LineTab T, int nlanes, long long si, long long so,
+            long long sl, double ih) {
+  constexpr int NV = Prob::NV;
+  constexpr int PMAX = 1024 / LG;
+  extern __shared__ double lds[];
+  const int n = T.n, P = T.P, K = T.K, KB = T.KB;
+  const int ntab = 5 * n + 1;
+  double* s_ib = lds;
+  double* s_m = s_ib + n;
+  double* s_gam = s_m + n;
+  double* s_H = s_gam + n + 1;
+  double* s_G = s_H + n;
+  double* s_wf = s_G + n;         // [P*KB]
+  double* s_wb = s_wf + P * KB;   // [P*KB]
+  double* s_bt = s_wb + P * KB;   // [7n]
+  double* sh = s_bt + 7 * n;      // [PMAX][LG] carries
+  double* tile = sh + PMAX * LG;  // [n][LG]
+  const int tid = threadIdx.x;
+  const int li = tid % LG;
+  const int c = tid / LG;
+  const int i = blockIdx.x * LG + li;
+  const bool act = i < nlanes && c < P;
+  const int ic = i < nlanes ? i : nlanes - 1;
+  const long long base = (long long)ic * si + (long long)blockIdx.y * so;
+  const int s = c * C;
+
+  double v[C][NV];
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    const int l = s + q < n ? s + q : n - 1;
+    prob.load(base + (long long)l * sl, v[q]);
+  }
+  {
+    const int nstage = ntab + 2 * P * KB;
+    const int woff = (int)(T.WfB - T.blob);
+    for (int t = tid; t < nstage; t += blockDim.x)
+      lds[t] = T.blob[t < ntab ? t : t - ntab + woff];
+    for (int t = tid; t < 7 * n; t += blockDim.x) s_bt[t] = T.BT[t];
+  }
+#pragma unroll
+  for (int a = 0; a < NV; a++) {
+    __syncthreads();
+    double y[C];
+    double yp = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int l = s + q;
+      if (act && l < n) {
+        yp = v[q][a] * s_ib[l] - s_m[l] * yp;
+        y[q] = yp;
+      } else {
+        y[q] = 0.0;
+      }
+    }
+    sh[c * LG + li] = act ? yp : 0.0;
+    __syncthreads();
+    if (act) {
+      const int kk = c < K ? c : K;
+      double Y = 0.0;
+      for (int k = 0; k < kk; k++) Y += s_wf[c * KB + k] * sh[(c - 1 - k) * LG + li];
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (l < n) y[q] += s_H[l] * Y;
+      }
+    }
+    double xs = 0.0;
+#pragma unroll
+    for (int q = C - 1; q >= 0; q--) {
+      const int l = s + q;
+      if (act && l < n) {
+        xs = y[q] - s_gam[l + 1] * xs;
+        y[q] = xs;
+      }
+    }
+    __syncthreads();
+    sh[c * LG + li] = act ? xs : 0.0;
+    __syncthreads();
+    if (act) {
+      const int kb = P - 1 - c < K ? P - 1 - c : K;
+      double X = 0.0;
+      for (int k = 0; k < kb; k++) X += s_wb[c * KB + k] * sh[(c + 1 + k) * LG + li];
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (l < n) tile[l * LG + li] = y[q] + s_G[l] * X;
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (l < n) {
+          double r = 0.0;
+#pragma unroll
+          for (int k = 0; k < 7; k++) {
+            const int t = l + k - 3;
+            if (t >= 0 && t < n) r += s_bt[7 * l + k] * tile[t * LG + li];
+          }
+          v[q][a] = ih * r;
+        }
+      }
+    }
+  }
+  if (!act) return;  // after the last barrier
+#pragma unroll
+  for (int q = 0; q < C; q++) {
+    const int l = s + q;
+    if (l < n) prob.store(base + (long long)l * sl, v[q]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host-side dispatch over (scheme, kind, C) for one direction of a grid.
 
 // Geometry of direction dir of an nx*ny*nz grid.
@@ -616,6 +741,7 @@ int launch_lines(emx_ctx* ctx, int dir, const Prob& prob, double ih) {
   const DirGeom g = dir_geom(ctx->d, dir);
   const int op = uses_tile<Prob>::value ? PLAN_FLUX : PLAN_DERIV;
   const LineTab& T = ctx->plan[op][dir].tab;
+  LaunchTimer timer(ctx, Prob::label(), dir, prob.bytes_pt() * (double)ctx->d.N());
   hipError_t e;
   switch (ctx->scheme) {
     case EMX_SCHEME_C4: e = launch_lines_s<EMX_SCHEME_C4>(prob, T, g, ih, ctx->stream); break;
@@ -625,6 +751,43 @@ int launch_lines(emx_ctx* ctx, int dir, const Prob& prob, double ih) {
   if (e != hipSuccess) {
     set_error("line-solver launch (dir %d, kind %d, C %d) failed: %s", dir,
               T.kind, T.C, hipGetErrorString(e));
+    return EMX_ERR_HIP;
+  }
+  return EMX_OK;
+}
+
+// Longest line of the transposed solve (its LDS tile holds a whole line of
+// 16 lanes).
+constexpr int ADJ_MAX_N = 512;
+
+// Chunk length of the transposed solve (16 lines per wavefront).
+inline int adj_chunk(int n) { return n <= 256 ? 4 : 8; }
+
+// Solve all lines of direction dir with the transposed operator; T is the
+// context's TAB_TRANSPOSED table of that direction.
+template <class Prob>
+int launch_lines_adj(emx_ctx* ctx, int dir, const Prob& prob, double ih) {
+  const DirGeom g = dir_geom(ctx->d, dir);
+  const LineTab& T = ctx->adj[dir];
+  LaunchTimer timer(ctx, Prob::label(), dir, prob.bytes_pt() * (double)ctx->d.N());
+  constexpr int LG = 16;
+  dim3 grid((g.nlanes + LG - 1) / LG, g.nouter);
+  const int threads = ((T.P * LG + 63) / 64) * 64;
+  const size_t lds = (size_t)(5 * T.n + 1 + 2 * T.P * T.KB + 7 * T.n + (1024 / LG) * LG +
+                              T.n * LG) * sizeof(double);
+  auto go = [&](auto kern) {
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)kern,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kern, grid, dim3(threads), lds, ctx->stream, prob, T, g.nlanes,
+                       g.si, g.so, g.sl, ih);
+  };
+  if (T.C == 4) go(k_lines_adj<4, LG, Prob>);
+  else go(k_lines_adj<8, LG, Prob>);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("transposed line-solver launch (dir %d, C %d) failed: %s", dir, T.C,
+              hipGetErrorString(e));
     return EMX_ERR_HIP;
   }
   return EMX_OK;

@@ -78,9 +78,19 @@ struct LineTab {
   const double* WbB;  // [P*KB] strided kinds: WbB[c*KB+k] = Wb[c][c+1+k]
   const double* Wf;   // [P*P]  forward carry weights (full)
   const double* Wb;   // [P*P]  backward carry weights (full)
+  // tables of the transposed matrix only (TAB_TRANSPOSED): B^T as seven
+  // diagonals, BT[7l+k] = B[l+k-3][l] (B = the right-hand-side matrix,
+  // inverse spacing excluded); nullptr otherwise
+  const double* BT;   // [7n]
 };
 
-// Builds (once per device) and returns the table.  Thread-safe.
+// Scheme flag of get_line_tab: the factors and carry weights of A^T
+// (sub- and super-diagonals exchanged) plus the B^T diagonals, for the
+// transposed (adjoint) derivative.
+constexpr int TAB_TRANSPOSED = 16;
+
+// Builds (once per device) and returns the table.  Thread-safe.  scheme
+// may carry TAB_TRANSPOSED.
 int get_line_tab(int scheme, int n, int C, int kind, LineTab* out);
 // Host copy of the Thomas factors (for tests and the adjoint).
 int host_thomas(int scheme, int n, std::vector<double>& a,
@@ -102,6 +112,22 @@ struct Dims {
   __host__ __device__ long long N() const { return (long long)nx * ny * nz; }
 };
 
+// Per-launch timing (emx_timing_begin / emx_timing_end): two HIP events on
+// the context stream around each launch, with the launch's algorithmic
+// bytes (every field it must read or write once, 8 B per point).
+struct TimedLaunch {
+  const char* label;
+  int dir;
+  double bytes;
+};
+struct TimingEntry {
+  const char* label;
+  int dir;
+  int launches;
+  double ms;
+  double bytes;
+};
+
 }  // namespace emx
 
 // The opaque context of the C ABI.
@@ -111,6 +137,9 @@ struct emx_ctx {
   emx::Dims d{0, 0, 0};
   hipStream_t stream = nullptr;
   emx::DirPlan plan[emx::PLAN_OPS][3];  // [operation class][direction]
+  // transposed-operator tables per direction (empty when a line is longer
+  // than the transposed solve supports)
+  emx::LineTab adj[3] = {};
   // persistent scratch (device): eight accumulators for the fused
   // flux-divergence, four fields for the eddy-viscosity kernels.
   double* acc = nullptr;       // 8*N
@@ -121,4 +150,31 @@ struct emx_ctx {
   // staging for the host-pointer entry points
   double* stage = nullptr;
   size_t stage_bytes = 0;
+  // per-launch timing (off unless emx_timing_begin)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;          // 2 per recorded launch
+  std::vector<emx::TimedLaunch> timed;
+  std::vector<emx::TimingEntry> timing_summary;
 };
+
+namespace emx {
+// Brackets one kernel launch with timing events when the context records.
+class LaunchTimer {
+ public:
+  LaunchTimer(emx_ctx* c, const char* label, int dir, double bytes);
+  ~LaunchTimer();
+  LaunchTimer(const LaunchTimer&) = delete;
+  LaunchTimer& operator=(const LaunchTimer&) = delete;
+
+ private:
+  emx_ctx* c_;
+  bool on_;
+};
+}  // namespace emx
+
+// One pointwise launch statement, timed when the context records.
+#define EMX_TIMED(ctx, label, bytes, ...)                      \
+  do {                                                         \
+    ::emx::LaunchTimer timer_((ctx), (label), -1, (bytes));    \
+    __VA_ARGS__;                                               \
+  } while (0)

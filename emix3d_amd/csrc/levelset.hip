@@ -307,26 +307,36 @@ int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
   double* D2 = D1 + NG;
   hipStream_t st = ctx->stream;
   const unsigned bg = blocks_for(NG);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, pres, d.nx, d.ny, d.nz, P, g);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, sobject, d.nx, d.ny, d.nz, S, g);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g);
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, pres, d.nx, d.ny, d.nz, P, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, sobject, d.nx, d.ny, d.nz, S, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g));
   // cuExtrapolation(d_press, d_sobject, d_jbn, deltaX.., NxG.., deltamin*0.5, 1)
   // hands 1.0/deltaX to every kernel (interface.cu:373-421)
   const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz, dt = deltamin * 0.5;
-  hipLaunchKernelGGL(k_ls_grad1, dim3(bg), dim3(256), 0, st, S, J1, J5, J9, g,
-                     iX, iY, iZ, dphi);
+  EMX_TIMED(ctx, "ls_grad1", 56.0 * NG,
+          hipLaunchKernelGGL(k_ls_grad1, dim3(bg), dim3(256), 0, st, S, J1, J5, J9, g,
+                     iX, iY, iZ, dphi));
   for (int it = 1; it <= 10; it++) {
-    hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 0, P, P,
-                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D1);
-    hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 1, D1, P,
-                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D2);
-    hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 2, D2, P,
-                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, P);
+    EMX_TIMED(ctx, "ls_extrap_stage", 72.0 * NG,
+          hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 0, P, P,
+                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D1));
+    EMX_TIMED(ctx, "ls_extrap_stage", 80.0 * NG,
+          hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 1, D1, P,
+                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D2));
+    EMX_TIMED(ctx, "ls_extrap_stage", 80.0 * NG,
+          hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 2, D2, P,
+                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, P));
   }
-  hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
-                     d.nx, d.ny, d.nz, pres);
+  EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
+          hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
+                     d.nx, d.ny, d.nz, pres));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }
@@ -346,22 +356,31 @@ int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
   double* T2 = T1 + NG;
   hipStream_t st = ctx->stream;
   const unsigned bg = blocks_for(NG);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, phi, d.nx, d.ny, d.nz, P, g);
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, phi, d.nx, d.ny, d.nz, P, g));
   for (int c = 0; c < 3; c++)
-    hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, vel + c * N, d.nx, d.ny,
-                       d.nz, V + c * NG, g);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g);
-  hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g);
+    EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, vel + c * N, d.nx, d.ny,
+                       d.nz, V + c * NG, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g));
   const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz;
-  hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, V, J1,
-                     J5, J9, g, iX, iY, iZ, dt, T1);
-  hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, V, J1,
-                     J5, J9, g, iX, iY, iZ, dt, T2);
-  hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, V, J1,
-                     J5, J9, g, iX, iY, iZ, dt, P);
-  hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
-                     d.nx, d.ny, d.nz, phi);
+  EMX_TIMED(ctx, "ls_weno_stage", 64.0 * NG,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, V, J1,
+                     J5, J9, g, iX, iY, iZ, dt, T1));
+  EMX_TIMED(ctx, "ls_weno_stage", 72.0 * NG,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, V, J1,
+                     J5, J9, g, iX, iY, iZ, dt, T2));
+  EMX_TIMED(ctx, "ls_weno_stage", 72.0 * NG,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, V, J1,
+                     J5, J9, g, iX, iY, iZ, dt, P));
+  EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
+          hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
+                     d.nx, d.ny, d.nz, phi));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -11,7 +11,7 @@
 #include <cstdio>
 #include <cstring>
 
-#include "emx_internal.hpp"
+#include "compact.hpp"
 
 namespace emx {
 
@@ -72,9 +72,40 @@ struct TabKey {
 std::mutex g_tab_mu;
 std::map<TabKey, LineTab> g_tabs;  // device memory held until process exit
 
-int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
+// B^T diagonals of scheme S: column l of B from the right-hand-side rule
+// applied to unit vectors.
+template <int S>
+void rhs_transposed(int n, std::vector<double>& BT) {
+  BT.assign((size_t)7 * n, 0.0);
+  for (int l = 0; l < n; l++)
+    for (int k = 0; k < 7; k++) {
+      const int r = l + k - 3;  // row of B
+      if (r < 0 || r >= n) continue;
+      double w[7] = {0, 0, 0, 0, 0, 0, 0};
+      w[3 + (l - r)] = 1.0;  // f_l is w[l-r] of row r
+      BT[(size_t)7 * l + k] = rhs_row<S>(w + 3, r, n, 1.0);
+    }
+}
+
+int build_tab(int scheme_flags, int n, int C, int kind, LineTab* t) {
+  const int scheme = scheme_flags & ~TAB_TRANSPOSED;
+  const bool transposed = (scheme_flags & TAB_TRANSPOSED) != 0;
   std::vector<double> a, b, c;
   EMX_TRY(host_thomas(scheme, n, a, b, c));
+  std::vector<double> BT;
+  if (transposed) {
+    // row l of A^T: A[l-1][l], A[l][l], A[l+1][l] = c_{l-1}, b_l, a_{l+1}
+    std::vector<double> at(n, 0.0), ct(n, 0.0);
+    for (int l = 1; l < n; l++) at[l] = c[l - 1];
+    for (int l = 0; l + 1 < n; l++) ct[l] = a[l + 1];
+    a.swap(at);
+    c.swap(ct);
+    switch (scheme) {
+      case EMX_SCHEME_C4: rhs_transposed<EMX_SCHEME_C4>(n, BT); break;
+      case EMX_SCHEME_C6F: rhs_transposed<EMX_SCHEME_C6F>(n, BT); break;
+      default: rhs_transposed<EMX_SCHEME_C6C>(n, BT); break;
+    }
+  }
   const int P = (n + C - 1) / C;
   const int pmax = kind == KIND_WAVE ? 64 : kind == KIND_STRIDED ? 16
                  : kind == KIND_STRIDED32 ? 32 : 64;
@@ -153,7 +184,7 @@ int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
     }
   // one device block, in the order the kernels stage it into LDS:
   //   ib m gam(n+1) H G | Af Ab (wave kind) | WfB WbB (strided kinds) | Wf Wb
-  size_t off[12];
+  size_t off[13];
   off[0] = 0;
   off[1] = off[0] + n;                 // m
   off[2] = off[1] + n;                 // gam (n+1)
@@ -165,8 +196,9 @@ int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
   off[8] = off[7] + (size_t)P * KB;    // WbB
   off[9] = off[8] + (size_t)P * KB;    // Wf
   off[10] = off[9] + (size_t)P * P;    // Wb
-  off[11] = off[10] + (size_t)P * P;
-  std::vector<double> host(off[11]);
+  off[11] = off[10] + (size_t)P * P;  // BT (transposed tables)
+  off[12] = off[11] + BT.size();
+  std::vector<double> host(off[12]);
   memcpy(&host[off[0]], ib.data(), n * sizeof(double));
   memcpy(&host[off[1]], m.data(), n * sizeof(double));
   memcpy(&host[off[2]], gam.data(), (n + 1) * sizeof(double));
@@ -178,12 +210,13 @@ int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
   memcpy(&host[off[8]], WbB.data(), (size_t)P * KB * sizeof(double));
   memcpy(&host[off[9]], Wf.data(), (size_t)P * P * sizeof(double));
   memcpy(&host[off[10]], Wb.data(), (size_t)P * P * sizeof(double));
+  if (!BT.empty()) memcpy(&host[off[11]], BT.data(), BT.size() * sizeof(double));
   double* dev = nullptr;
-  if (hipMalloc(&dev, off[11] * sizeof(double)) != hipSuccess) {
-    set_error("hipMalloc of a %zu-byte line table failed", off[11] * sizeof(double));
+  if (hipMalloc(&dev, off[12] * sizeof(double)) != hipSuccess) {
+    set_error("hipMalloc of a %zu-byte line table failed", off[12] * sizeof(double));
     return EMX_ERR_NOMEM;
   }
-  EMX_TRY_HIP(hipMemcpy(dev, host.data(), off[11] * sizeof(double),
+  EMX_TRY_HIP(hipMemcpy(dev, host.data(), off[12] * sizeof(double),
                         hipMemcpyHostToDevice));
   t->n = n;
   t->C = C;
@@ -203,6 +236,7 @@ int build_tab(int scheme, int n, int C, int kind, LineTab* t) {
   t->WbB = dev + off[8];
   t->Wf = dev + off[9];
   t->Wb = dev + off[10];
+  t->BT = BT.empty() ? nullptr : dev + off[11];
   return EMX_OK;
 }
 
@@ -223,6 +257,40 @@ int get_line_tab(int scheme, int n, int C, int kind, LineTab* out) {
   g_tabs[key] = t;
   *out = t;
   return EMX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-launch timing.  Events are created on demand while the context
+// records and reused by later recordings; a failure to create or record
+// one stops the recording (timing is a diagnostic, never an error of the
+// operator).
+LaunchTimer::LaunchTimer(emx_ctx* c, const char* label, int dir, double bytes)
+    : c_(c), on_(c && c->timing) {
+  if (!on_) return;
+  const size_t need = 2 * (c->timed.size() + 1);
+  while (c->ev_pool.size() < need) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      c->timing = false;
+      on_ = false;
+      return;
+    }
+    c->ev_pool.push_back(e);
+  }
+  if (hipEventRecord(c->ev_pool[need - 2], c->stream) != hipSuccess) {
+    c->timing = false;
+    on_ = false;
+    return;
+  }
+  c->timed.push_back(TimedLaunch{label, dir, bytes});
+}
+
+LaunchTimer::~LaunchTimer() {
+  if (!on_) return;
+  if (hipEventRecord(c_->ev_pool[2 * c_->timed.size() - 1], c_->stream) != hipSuccess) {
+    c_->timed.pop_back();
+    c_->timing = false;
+  }
 }
 
 }  // namespace emx

@@ -32,6 +32,17 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
             const double* jbn, const double* sgd, double c4, double* amut,
             double* vis, double* vis5);
 
+// adjoints (ops_adj.hip)
+int op_deriv_adj(emx_ctx* ctx, int dir, const double* g, double* out, double ih);
+int op_grad_adj(emx_ctx* ctx, const double* dvel_bar, const double* dtemp_bar,
+                double* u_bar, double* temp_bar, const double ih[3]);
+int op_divergence_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_bar,
+                      const double* jbn, const double ih[3], double* flux_bar);
+int op_flux_div_all_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_bar,
+                        const FluxPtrs& fp, const double ih[3], double* um_bar,
+                        double* u_bar, double* pres_bar, double* vis_bar,
+                        double* dvel_bar);
+
 // level set (levelset.hip); scratch is ctx->ls_buf
 size_t ls_scratch_doubles(const Dims& d);
 int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,

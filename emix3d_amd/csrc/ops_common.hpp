@@ -10,12 +10,17 @@
 namespace emx {
 
 // ---------------------------------------------------------------------------
-// Problems for the line solver.
+// Problems for the line solver.  Besides load/store each problem names
+// itself (label) and states its algorithmic bytes per grid point
+// (bytes_pt: every field the launch must read or write once, 8 B each),
+// which the per-launch timing reports.
 
 // NV independent fields in, NV derivatives out.
 template <int N_>
 struct ProbDeriv {
   static constexpr int NV = N_;
+  static const char* label() { return "deriv"; }
+  double bytes_pt() const { return 16.0 * N_; }
   const double* in[N_];
   double* out[N_];
   __device__ __forceinline__ void load(long long p, double* v) const {
@@ -88,6 +93,8 @@ struct FluxArr {
   long long N;
   __device__ __forceinline__ double conv(int dir, long long p) const { return flux[dir * N + p]; }
   __device__ __forceinline__ double visc(int dir, long long p) const { return flux[(3 + dir) * N + p]; }
+  static const char* label() { return "divergence"; }
+  int in_fields(int, bool visc) const { return visc ? 2 : 1; }
 };
 
 // Fluxes formed on the fly from the primitive inputs.
@@ -96,6 +103,17 @@ struct FluxLive {
   int neq;
   __device__ __forceinline__ double conv(int dir, long long p) const { return conv_flux(in, neq, dir, p); }
   __device__ __forceinline__ double visc(int dir, long long p) const { return visc_flux(in, neq, dir, p); }
+  static const char* label() { return "flux_div"; }
+  // distinct input fields of the fluxes of neq along dir, besides the
+  // metric of dir: um(2+dir) [neq 1]; u(1+dir), um5 [neq 5]; um(2+dir),
+  // u(neq-1), pres on the diagonal, and for the viscous flux vis plus one
+  // dvel (diagonal) or one more metric and two dvel [neq 2..4]
+  int in_fields(int dir, bool visc) const {
+    if (neq == 1) return 1;
+    if (neq == 5) return 2;
+    const bool diag = neq - 2 == dir;
+    return 2 + (diag ? 1 : 0) + (visc ? (diag ? 2 : 4) : 0);
+  }
 };
 
 // Projection E = J*(xi_x*e) (solver/divergencia.f90:228-240) fused with
@@ -113,6 +131,12 @@ struct ProbDiv {
   double* accv;
   double* rs;
   double* rsv;
+  static const char* label() { return Src::label(); }
+  // fluxes + J + metric (+ 1/J in the last pass), accumulators
+  double bytes_pt() const {
+    const int in = src.in_fields(dir, VISC) + 2 + (pass == 2 ? 1 : 0);
+    return 8.0 * (in + (pass > 0 ? NV : 0) + NV);
+  }
   __device__ __forceinline__ void load(long long p, double* v) const {
     const double det = jbn[9 * N + p];
     const double met = jbn[(4 * dir) * N + p];  // comps 1, 5, 9
@@ -151,6 +175,17 @@ struct ProbDivAll {
   double* acc;  // 8 fields
   double* rs;   // 5 fields
   double* rsv;  // 5 fields
+  static const char* label() {
+    return G == 0 ? "flux_div_all[conv1-4]"
+                  : (G == 1 ? "flux_div_all[conv5,visc2-4]" : "flux_div_all[8]");
+  }
+  // inputs along dir: G=0 um(2+dir), u(1..3), pres, J, metric;
+  // G=1 u(1+dir), um5, vis, jbn 1/5/9, five dvel, J; G=2 the union (16);
+  // 1/J in the last pass; accumulators
+  double bytes_pt() const {
+    const int in = (G == 0 ? 7 : (G == 1 ? 12 : 16)) + (pass == 2 ? 1 : 0);
+    return 8.0 * (in + (pass > 0 ? NV : 0) + NV);
+  }
   __device__ __forceinline__ double val(int slot, long long p) const {
     return slot < 5 ? conv_flux(in, slot + 1, dir, p) : visc_flux(in, slot - 3, dir, p);
   }

@@ -29,8 +29,11 @@ __global__ void k_flujos(FluxIn F, int neq, double* flux) {
 
 int op_flux(emx_ctx* ctx, int neq, const FluxPtrs& fp, double* flux) {
   FluxIn F{fp.um, fp.u, fp.pres, fp.vis, fp.jbn, fp.dvel, ctx->d.N()};
-  hipLaunchKernelGGL(k_flujos, dim3(pointwise_blocks(F.N)), dim3(256), 0,
-                     ctx->stream, F, neq, flux);
+  // inputs: um 2..4 [neq 1]; um 2..4, u(neq-1), pres, vis, jbn 1/5/9 and
+  // five dvel [neq 2..4]; u 1..3, um5 [neq 5]; outputs: 6 fields
+  EMX_TIMED(ctx, "flujos", 8.0 * (6 + (neq == 1 ? 3 : (neq == 5 ? 4 : 14))) * F.N,
+          hipLaunchKernelGGL(k_flujos, dim3(pointwise_blocks(F.N)), dim3(256), 0,
+                     ctx->stream, F, neq, flux));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }
@@ -70,8 +73,9 @@ int op_flux_div(emx_ctx* ctx, int neq, const FluxPtrs& fp, const double ih[3],
   EMX_TRY((div_passes<FluxLive, false>(ctx, src, fp.jbn, ih, rs, rsv)));
   // the viscous fluxes of the continuity and scalar equations are zero
   // (flujos_compact.f90:32-40,187-214), so their divergence is zero
-  hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
-                     ctx->stream, rsv, 0.0, N);
+  EMX_TIMED(ctx, "fill", 8.0 * N,
+          hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
+                     ctx->stream, rsv, 0.0, N));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }
@@ -95,10 +99,12 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
     }
   }
   // zero viscous divergence of the continuity and scalar equations
-  hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
-                     ctx->stream, rsv, 0.0, N);
-  hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
-                     ctx->stream, rsv + 4 * N, 0.0, N);
+  EMX_TIMED(ctx, "fill", 8.0 * N,
+          hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
+                     ctx->stream, rsv, 0.0, N));
+  EMX_TIMED(ctx, "fill", 8.0 * N,
+          hipLaunchKernelGGL(k_fill, dim3(pointwise_blocks(N)), dim3(256), 0,
+                     ctx->stream, rsv + 4 * N, 0.0, N));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

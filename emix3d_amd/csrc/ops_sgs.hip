@@ -173,11 +173,13 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
   dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, (d.nz + kz - 1) / kz);
   dim3 block(64, 4);
   double* w = ctx->sgs_tmp;
-  hipLaunchKernelGGL(k_sgs_structure, grid, block, 0, ctx->stream, u, dvel,
-                     jbn, sgd, d, kz, amut, w);
+  EMX_TIMED(ctx, "sgdm_structure", 184.0 * d.N(),
+          hipLaunchKernelGGL(k_sgs_structure, grid, block, 0, ctx->stream, u, dvel,
+                     jbn, sgd, d, kz, amut, w));
   EMX_TRY_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_sgs_select, grid, block, 0, ctx->stream, w, d, kz, c4,
-                     amut, vis, vis5);
+  EMX_TIMED(ctx, "sgdm_select", (vis5 ? 56.0 : 48.0) * d.N(),
+          hipLaunchKernelGGL(k_sgs_select, grid, block, 0, ctx->stream, w, d, kz, c4,
+                     amut, vis, vis5));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -78,6 +78,20 @@ int emx_set_stream(emx_ctx* ctx, void* stream);
  * Used by the autotuner. */
 int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C);
 int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C);
+
+/* Per-launch timing, for diagnostics and the launch-shape autotuner (no
+ * counterpart in the reference, which has no timers).  Between
+ * emx_timing_begin and emx_timing_end every kernel the context launches is
+ * bracketed by two HIP events on the context stream (events are created on
+ * first use and reused).  emx_timing_end waits for the last launch and
+ * aggregates per (kernel label, direction), sorted by total time; entry i
+ * gives the label, direction (-1 for pointwise kernels), launch count,
+ * summed duration and summed algorithmic bytes (each field the kernel must
+ * read or write once, 8 B per grid point). */
+int emx_timing_begin(emx_ctx* ctx);
+int emx_timing_end(emx_ctx* ctx, int* n_entries);
+int emx_timing_entry(const emx_ctx* ctx, int i, const char** label, int* dir,
+                     int* launches, double* total_ms, double* bytes);
 int emx_sync(emx_ctx* ctx);
 
 /* ---- device-pointer operator entry points ------------------------------- */
@@ -139,6 +153,39 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
                      const double* pres, const double* vis, const double* jbn,
                      const double* dvel, double ihx, double ihy, double ihz,
                      double* rs, double* rsv);
+
+/* ---- adjoints (transposes) ------------------------------------------------
+ * No counterpart in the reference, which has no adjoint.  Each entry is the
+ * exact transpose of the forward entry it names (same scheme and metric
+ * projection), for gradient-based use and the learned closure's training.
+ * Lines of at most 512 points in every direction (EMX_ERR_SIZE otherwise).
+ * Outputs are overwritten. */
+
+/* out = D^T g for D = emx_deriv along dir with inverse spacing inv_h. */
+int emx_deriv_adj(emx_ctx* ctx, int dir, const double* g, double* out,
+                  double inv_h);
+
+/* Transpose of emx_grad: u_bar(c) = sum_d D_d^T dvel_bar(3d+c) (0-based
+ * components), temp_bar = sum_d D_d^T dtemp_bar(d); the temperature pair
+ * may both be NULL. */
+int emx_grad_adj(emx_ctx* ctx, const double* dvel_bar, const double* dtemp_bar,
+                 double* u_bar, double* temp_bar, double ihx, double ihy,
+                 double ihz);
+
+/* Transpose of emx_divergence with respect to its six flux fields. */
+int emx_divergence_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_bar,
+                       const double* jbn, double ihx, double ihy, double ihz,
+                       double* flux_bar);
+
+/* Transpose of the linearisation of emx_flux_div_all about (um, u, vis,
+ * dvel) (pres enters linearly; jbn is fixed): from the adjoints of rs (5
+ * fields) and rsv (5 fields; 1 and 5 unused) to those of um (5, the first
+ * is zero), u (3), pres, vis and dvel (9). */
+int emx_flux_div_all_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_bar,
+                         const double* um, const double* u, const double* vis,
+                         const double* jbn, const double* dvel, double ihx,
+                         double ihy, double ihz, double* um_bar, double* u_bar,
+                         double* pres_bar, double* vis_bar, double* dvel_bar);
 
 /* Pressure extrapolation into the solid (sobject > 0): advect_CUDA
  * (src/LevelSet/lsTransportEc.cu:24-197 -> interface.cu:341-436).

@@ -903,3 +903,243 @@ int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gp,
     free(w);
     return 0;
 }
+
+
+/* ---------------------------------------------------------------------- */
+/* Adjoints.  The reference has no adjoint; these are the transposes of   */
+/* the functions above, written from D = inv_h * A^{-1} B:                */
+/*   D^T g = B^T A^{-T} g  (B including inv_h, as rhs_line forms it).     */
+/* ---------------------------------------------------------------------- */
+
+/* Columns of B: BT[7*l + k] = B[l+k-3][l], from rhs_line applied to the
+ * unit vectors (n calls, once per 3-D call). */
+static double *rhs_columns(int scheme, int n, double ih)
+{
+    double *BT = (double *)calloc((size_t)7 * n, sizeof(double));
+    double *e = (double *)calloc((size_t)n, sizeof(double));
+    double *r = (double *)malloc(sizeof(double) * (size_t)n);
+    if (!BT || !e || !r) { free(BT); free(e); free(r); return NULL; }
+    for (int l = 0; l < n; l++) {
+        e[l] = 1.0;
+        rhs_line(scheme, n, e, 1, ih, r);
+        for (int k = 0; k < 7; k++) {
+            int row = l + k - 3;
+            if (row >= 0 && row < n) BT[7 * l + k] = r[row];
+        }
+        e[l] = 0.0;
+    }
+    free(e);
+    free(r);
+    return BT;
+}
+
+/* One line: solve A^T y = g (Thomas on the transposed tridiagonal: row l
+ * is (c_{l-1}, b_l, a_{l+1})), then out = B^T y. */
+static void deriv_line_adj(int n, const double *a, const double *b,
+                           const double *c, const double *BT, const double *g,
+                           long is, double *out, long os, double *y, double *gam)
+{
+    double bet = b[0];
+    y[0] = g[0] / bet;
+    for (int l = 1; l < n; l++) {
+        double at = c[l - 1], ctm = a[l];   /* A^T[l][l-1], A^T[l-1][l] */
+        gam[l] = ctm / bet;
+        bet = b[l] - at * gam[l];
+        y[l] = (g[(long)l * is] - at * y[l - 1]) / bet;
+    }
+    for (int l = n - 2; l >= 0; l--) y[l] = y[l] - gam[l + 1] * y[l + 1];
+    for (int l = 0; l < n; l++) {
+        double s = 0.0;
+        for (int k = 0; k < 7; k++) {
+            int t = l + k - 3;
+            if (t >= 0 && t < n) s += BT[7 * l + k] * y[t];
+        }
+        out[(long)l * os] = s;
+    }
+}
+
+int orc_deriv3d_adj(int scheme, int dir, int nx, int ny, int nz, const double *g,
+                    double inv_h, double *out)
+{
+    int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    if (n < scheme_min_n(scheme)) return -1;
+    double *buf = (double *)malloc(sizeof(double) * 5 * (size_t)n);
+    double *BT = rhs_columns(scheme, n, inv_h);
+    if (!buf || !BT) { free(buf); free(BT); return -3; }
+    double *a = buf, *b = buf + n, *c = buf + 2 * n, *y = buf + 3 * n, *gam = buf + 4 * n;
+    orc_coeffs(scheme, n, a, b, c);
+    long sx = 1, sy = nx, sz = (long)nx * ny;
+    if (dir == 0) {
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                deriv_line_adj(n, a, b, c, BT, g + IDX(0, j, k), sx, out + IDX(0, j, k), sx, y, gam);
+    } else if (dir == 1) {
+        for (int k = 0; k < nz; k++)
+            for (int i = 0; i < nx; i++)
+                deriv_line_adj(n, a, b, c, BT, g + IDX(i, 0, k), sy, out + IDX(i, 0, k), sy, y, gam);
+    } else {
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++)
+                deriv_line_adj(n, a, b, c, BT, g + IDX(i, j, 0), sz, out + IDX(i, j, 0), sz, y, gam);
+    }
+    free(buf);
+    free(BT);
+    return 0;
+}
+
+int orc_deriv_vel_adj(int scheme, int nx, int ny, int nz, const double *dvel_bar,
+                      const double *dtemp_bar, const double ih[3], double *u_bar,
+                      double *temp_bar)
+{
+    long N = (long)nx * ny * nz;
+    double *t = (double *)malloc(sizeof(double) * (size_t)N);
+    if (!t) return -3;
+    memset(u_bar, 0, sizeof(double) * 3 * (size_t)N);
+    if (dtemp_bar) memset(temp_bar, 0, sizeof(double) * (size_t)N);
+    for (int d = 0; d < 3; d++) {
+        for (int c = 0; c < 3; c++) {
+            if (orc_deriv3d_adj(scheme, d, nx, ny, nz, dvel_bar + (3 * d + c) * N, ih[d], t)) {
+                free(t);
+                return -1;
+            }
+            for (long p = 0; p < N; p++) u_bar[c * N + p] += t[p];
+        }
+        if (dtemp_bar) {
+            if (orc_deriv3d_adj(scheme, d, nx, ny, nz, dtemp_bar + d * N, ih[d], t)) {
+                free(t);
+                return -1;
+            }
+            for (long p = 0; p < N; p++) temp_bar[p] += t[p];
+        }
+    }
+    free(t);
+    return 0;
+}
+
+int orc_divergencia_adj(int scheme, int nx, int ny, int nz, const double *rs_bar,
+                        const double *rsv_bar, const double *jbn, double ihx,
+                        double ihy, double ihz, double *flux_bar)
+{
+    long N = (long)nx * ny * nz;
+    static const int mcomp[3] = {1, 5, 9};
+    double ih[3] = {ihx, ihy, ihz};
+    double *w = (double *)malloc(sizeof(double) * (size_t)N);
+    double *d = (double *)malloc(sizeof(double) * (size_t)N);
+    if (!w || !d) { free(w); free(d); return -3; }
+    const double *ij = jbn + 10 * N, *det = jbn + 9 * N;
+    int rc = 0;
+    for (int q = 0; q < 6 && !rc; q++) {
+        int dir = q % 3;
+        const double *src = q < 3 ? rs_bar : rsv_bar;
+        const double *met = jbn + (mcomp[dir] - 1) * N;
+        for (long p = 0; p < N; p++) w[p] = ij[p] * src[p];
+        rc = orc_deriv3d_adj(scheme, dir, nx, ny, nz, w, ih[dir], d);
+        for (long p = 0; p < N && !rc; p++) flux_bar[q * N + p] = det[p] * met[p] * d[p];
+    }
+    free(w);
+    free(d);
+    return rc;
+}
+
+int orc_flujos_adj(int neq, int nx, int ny, int nz, const double *um,
+                   const double *u, const double *vis, const double *jbn,
+                   const double *dvel, const double *flux_bar, double *um_bar,
+                   double *u_bar, double *pres_bar, double *vis_bar,
+                   double *dvel_bar)
+{
+    long N = (long)nx * ny * nz;
+    if (neq < 1 || neq > 5) return -1;
+    for (long p = 0; p < N; p++) {
+#define UM(m) um[((m) - 1) * N + p]
+#define U(m) u[((m) - 1) * N + p]
+#define J(c) jbn[((c) - 1) * N + p]
+#define D(c) dvel[((c) - 1) * N + p]
+#define UMB(m) um_bar[((m) - 1) * N + p]
+#define UB(m) u_bar[((m) - 1) * N + p]
+#define DB(c) dvel_bar[((c) - 1) * N + p]
+        const double eb = flux_bar[p], fb = flux_bar[N + p], gb = flux_bar[2 * N + p];
+        const double evb = flux_bar[3 * N + p], fvb = flux_bar[4 * N + p],
+                     gvb = flux_bar[5 * N + p];
+        const double mu = vis[p];
+        switch (neq) {
+        case 1: /* e,f,g = -um2,-um3,-um4 */
+            UMB(2) -= eb; UMB(3) -= fb; UMB(4) -= gb;
+            break;
+        case 2: case 3: case 4: {
+            int a = neq - 1; /* velocity component 1..3 */
+            double cb[3] = {eb, fb, gb};
+            for (int dd = 0; dd < 3; dd++) {
+                UMB(2 + dd) -= cb[dd] * U(a);
+                UB(a) -= cb[dd] * UM(2 + dd);
+            }
+            pres_bar[p] -= cb[a - 1];
+            if (neq == 2) {
+                /* ev = mu*2*J1*D1; fv = mu*(J1*D2+J5*D4); gv = mu*(J1*D3+J9*D7) */
+                vis_bar[p] += evb * (2.0 * (J(1) * D(1))) + fvb * (J(1) * D(2) + J(5) * D(4))
+                            + gvb * (J(1) * D(3) + J(9) * D(7));
+                DB(1) += evb * mu * 2.0 * J(1);
+                DB(2) += fvb * mu * J(1); DB(4) += fvb * mu * J(5);
+                DB(3) += gvb * mu * J(1); DB(7) += gvb * mu * J(9);
+            } else if (neq == 3) {
+                /* ev = mu*(J1*D2+J5*D4); fv = mu*2*J5*D5; gv = mu*(J9*D8+J5*D6) */
+                vis_bar[p] += evb * (J(1) * D(2) + J(5) * D(4)) + fvb * (2.0 * (J(5) * D(5)))
+                            + gvb * (J(9) * D(8) + J(5) * D(6));
+                DB(2) += evb * mu * J(1); DB(4) += evb * mu * J(5);
+                DB(5) += fvb * mu * 2.0 * J(5);
+                DB(8) += gvb * mu * J(9); DB(6) += gvb * mu * J(5);
+            } else {
+                /* ev = mu*(J1*D3+J9*D7); fv = mu*(J9*D8+J5*D6); gv = mu*2*J9*D9 */
+                vis_bar[p] += evb * (J(1) * D(3) + J(9) * D(7)) + fvb * (J(9) * D(8) + J(5) * D(6))
+                            + gvb * (2.0 * (J(9) * D(9)));
+                DB(3) += evb * mu * J(1); DB(7) += evb * mu * J(9);
+                DB(8) += fvb * mu * J(9); DB(6) += fvb * mu * J(5);
+                DB(9) += gvb * mu * 2.0 * J(9);
+            }
+            break;
+        }
+        case 5: { /* e,f,g = -u1*um5, -u2*um5, -u3*um5 */
+            double cb[3] = {eb, fb, gb};
+            for (int dd = 0; dd < 3; dd++) {
+                UB(1 + dd) -= cb[dd] * UM(5);
+                UMB(5) -= cb[dd] * U(1 + dd);
+            }
+            break;
+        }
+        }
+#undef UM
+#undef U
+#undef J
+#undef D
+#undef UMB
+#undef UB
+#undef DB
+    }
+    return 0;
+}
+
+int orc_flux_div_all_adj(int scheme, int nx, int ny, int nz, const double *rs_bar,
+                         const double *rsv_bar, const double *um, const double *u,
+                         const double *vis, const double *jbn, const double *dvel,
+                         double ihx, double ihy, double ihz, double *um_bar,
+                         double *u_bar, double *pres_bar, double *vis_bar,
+                         double *dvel_bar)
+{
+    long N = (long)nx * ny * nz;
+    double *fb = (double *)malloc(sizeof(double) * 6 * (size_t)N);
+    if (!fb) return -3;
+    memset(um_bar, 0, sizeof(double) * 5 * (size_t)N);
+    memset(u_bar, 0, sizeof(double) * 3 * (size_t)N);
+    memset(pres_bar, 0, sizeof(double) * (size_t)N);
+    memset(vis_bar, 0, sizeof(double) * (size_t)N);
+    memset(dvel_bar, 0, sizeof(double) * 9 * (size_t)N);
+    int rc = 0;
+    for (int m = 1; m <= 5 && !rc; m++) {
+        rc = orc_divergencia_adj(scheme, nx, ny, nz, rs_bar + (m - 1) * N,
+                                 rsv_bar + (m - 1) * N, jbn, ihx, ihy, ihz, fb);
+        if (!rc)
+            rc = orc_flujos_adj(m, nx, ny, nz, um, u, vis, jbn, dvel, fb, um_bar,
+                                u_bar, pres_bar, vis_bar, dvel_bar);
+    }
+    free(fb);
+    return rc;
+}

@@ -28,6 +28,10 @@
  *       convergence, 1-D upwind reduction).
  *   orc_dynsmag: no reference implementation exists (the reference's
  *       model is the structure-function sgdm); pinned only by closed forms.
+ *   orc_*_adj: the reference has no adjoint.  Each is the transpose of the
+ *       oracle function it names, pinned by dot-product tests against that
+ *       function (itself pinned as above) and by the transposed dense
+ *       operator matrix on short lines.
  */
 #ifndef EMX_ORACLE_H
 #define EMX_ORACLE_H
@@ -122,6 +126,41 @@ void orc_pad_mirror(int nx, int ny, int nz, int g, const double *in,
  * grid filter width delta (one value per point).  Clipped at nu_t >= 0. */
 int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gphys,
                 const double *delta, double *nut);
+
+/* ---- adjoints ----------------------------------------------------------- */
+
+/* out = D^T g along dir, D = orc_deriv3d(scheme, dir, ., inv_h). */
+int orc_deriv3d_adj(int scheme, int dir, int nx, int ny, int nz, const double *g,
+                    double inv_h, double *out);
+
+/* Transpose of orc_deriv_vel: u_bar(c) = sum_d D_d^T dvel_bar(3d+c);
+ * temp_bar = sum_d D_d^T dtemp_bar(d) (both may be NULL). */
+int orc_deriv_vel_adj(int scheme, int nx, int ny, int nz, const double *dvel_bar,
+                      const double *dtemp_bar, const double ih[3], double *u_bar,
+                      double *temp_bar);
+
+/* Transpose of orc_divergencia with respect to its 6 flux fields. */
+int orc_divergencia_adj(int scheme, int nx, int ny, int nz, const double *rs_bar,
+                        const double *rsv_bar, const double *jbn, double ihx,
+                        double ihy, double ihz, double *flux_bar);
+
+/* Transpose of the linearisation of orc_flujos(neq) about (um, u, vis,
+ * dvel); ADDS to the adjoint fields. */
+int orc_flujos_adj(int neq, int nx, int ny, int nz, const double *um,
+                   const double *u, const double *vis, const double *jbn,
+                   const double *dvel, const double *flux_bar, double *um_bar,
+                   double *u_bar, double *pres_bar, double *vis_bar,
+                   double *dvel_bar);
+
+/* Transpose of the linearisation of the five-equation flux divergence
+ * (orc_flujos + orc_divergencia, m = 1..5); rs_bar, rsv_bar: 5 fields each.
+ * Overwrites the adjoint fields. */
+int orc_flux_div_all_adj(int scheme, int nx, int ny, int nz, const double *rs_bar,
+                         const double *rsv_bar, const double *um, const double *u,
+                         const double *vis, const double *jbn, const double *dvel,
+                         double ihx, double ihy, double ihz, double *um_bar,
+                         double *u_bar, double *pres_bar, double *vis_bar,
+                         double *dvel_bar);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,12 @@ def lib():
             "orc_pad_mirror": [_I, _I, _I, _I, _P, _P],
             "orc_weno5_derivs": [_I, _I, _I, _P, _P, _D, _D, _D, _P, _P],
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
+            "orc_deriv3d_adj": [_I, _I, _I, _I, _I, _P, _D, _P],
+            "orc_deriv_vel_adj": [_I, _I, _I, _I, _P, _P, _P, _P, _P],
+            "orc_divergencia_adj": [_I, _I, _I, _I, _P, _P, _P, _D, _D, _D, _P],
+            "orc_flujos_adj": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P],
+            "orc_flux_div_all_adj": [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _D, _D, _D,
+                                     _P, _P, _P, _P, _P],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -98,6 +104,66 @@ def deriv3d(scheme, d, f, inv_h):
     out = np.empty_like(f)
     _check(lib().orc_deriv3d(scheme, d, nx, ny, nz, _ptr(f), inv_h, _ptr(out)), "deriv3d")
     return out
+
+
+def deriv3d_adj(scheme, d, g, inv_h):
+    """Transpose of deriv3d along d."""
+    nx, ny, nz = dims(g)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    out = np.empty_like(g)
+    _check(lib().orc_deriv3d_adj(scheme, d, nx, ny, nz, _ptr(g), inv_h, _ptr(out)),
+           "deriv3d_adj")
+    return out
+
+
+def deriv_vel_adj(scheme, dvel_bar, dtemp_bar, ih):
+    """Transpose of deriv_vel: (u_bar[3], temp_bar or None)."""
+    nx, ny, nz = dims(dvel_bar)
+    u_bar = np.empty((3, nz, ny, nx))
+    temp_bar = None if dtemp_bar is None else np.empty((nz, ny, nx))
+    ihs = (ctypes.c_double * 3)(*ih)
+    _check(lib().orc_deriv_vel_adj(scheme, nx, ny, nz, _ptr(dvel_bar),
+                                   None if dtemp_bar is None else _ptr(dtemp_bar),
+                                   ctypes.cast(ihs, _P), _ptr(u_bar),
+                                   None if temp_bar is None else _ptr(temp_bar)),
+           "deriv_vel_adj")
+    return u_bar, temp_bar
+
+
+def divergencia_adj(scheme, rs_bar, rsv_bar, jbn, ih):
+    """Transpose of divergencia with respect to the 6 flux fields."""
+    nx, ny, nz = dims(rs_bar)
+    fb = np.empty((6, nz, ny, nx))
+    _check(lib().orc_divergencia_adj(scheme, nx, ny, nz, _ptr(rs_bar), _ptr(rsv_bar),
+                                     _ptr(jbn), ih[0], ih[1], ih[2], _ptr(fb)),
+           "divergencia_adj")
+    return fb
+
+
+def flujos_adj(neq, um, u, vis, jbn, dvel, flux_bar, bars):
+    """Adds the transpose of flujos(neq)'s linearisation to bars (dict of
+    um_bar, u_bar, pres_bar, vis_bar, dvel_bar)."""
+    nx, ny, nz = dims(u)
+    _check(lib().orc_flujos_adj(neq, nx, ny, nz, _ptr(um), _ptr(u), _ptr(vis), _ptr(jbn),
+                                _ptr(dvel), _ptr(flux_bar), _ptr(bars["um_bar"]),
+                                _ptr(bars["u_bar"]), _ptr(bars["pres_bar"]),
+                                _ptr(bars["vis_bar"]), _ptr(bars["dvel_bar"])), "flujos_adj")
+    return bars
+
+
+def flux_div_all_adj(scheme, rs_bar, rsv_bar, um, u, vis, jbn, dvel, ih):
+    """Transpose of the five-equation flux divergence's linearisation."""
+    nx, ny, nz = dims(u)
+    bars = {"um_bar": np.empty((5, nz, ny, nx)), "u_bar": np.empty((3, nz, ny, nx)),
+            "pres_bar": np.empty((nz, ny, nx)), "vis_bar": np.empty((nz, ny, nx)),
+            "dvel_bar": np.empty((9, nz, ny, nx))}
+    _check(lib().orc_flux_div_all_adj(scheme, nx, ny, nz, _ptr(rs_bar), _ptr(rsv_bar),
+                                      _ptr(um), _ptr(u), _ptr(vis), _ptr(jbn), _ptr(dvel),
+                                      ih[0], ih[1], ih[2], _ptr(bars["um_bar"]),
+                                      _ptr(bars["u_bar"]), _ptr(bars["pres_bar"]),
+                                      _ptr(bars["vis_bar"]), _ptr(bars["dvel_bar"])),
+           "flux_div_all_adj")
+    return bars
 
 
 def jacobeano(scheme, x):
