@@ -10,8 +10,12 @@ viscosity), and flujos + divergencia for the five equations.  Inputs are
 resident in HBM when the timed region starts.
 
 Also reported, on the same JSON line:
-  roofline     -- the dominant kernel's achieved algorithmic bandwidth,
-                  timed with HIP events on the stream it runs on;
+  roofline     -- the dominant kernel's achieved algorithmic bandwidth: the
+                  library brackets every launch of the timed region with
+                  HIP events on its stream (emx_timing_begin/end), and the
+                  kernel with the largest total time is reported with its
+                  average launch duration and its algorithmic bytes per
+                  launch (each field it must read or write once);
   cpu_baseline -- the reference's own Fortran (oracle/_ref, built from the
                   reference sources by flang -O3) running the same stage
                   once on one host core (it is serial code);
@@ -43,7 +47,6 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--scheme", default="c6f")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--kernel-reps", type=int, default=20)
     return ap.parse_args()
 
 
@@ -79,17 +82,20 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    op.timing_begin()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    kernels = op.timing_end()
     pts = float(n) ** 3
     value = pts / dt
     log(f"bench: {dt * 1e3:.3f} ms per stage, {value / 1e9:.3f} Gpts/s")
-
-    # ---- dominant kernel, timed with events on the stream it runs on -----
-    roof = kernel_roofline(op, f, jbn, work, n, args.kernel_reps)
+    roof, breakdown = kernel_roofline(kernels, args.steps)
+    for b in breakdown:
+        log(f"bench:   {b['kernel']:44s} {b['us_per_launch']:8.1f} us x{b['launches_per_step']}"
+            f"  {b['GBs']:6.0f} GB/s")
 
     # ---- stage algorithmic bytes (each distinct input read once, each output
     # written once): inputs um(5) u(3) temp pres jbn(1,5,9,10,11) sgd(7) = 22,
@@ -123,38 +129,40 @@ def main():
                    "stage_algorithmic_GBs": stage_gbs,
                    "stage_frac_of_hbm_peak": stage_gbs / HBM_PEAK_GBS},
         "roofline": roof,
+        "kernels": breakdown,
         "cpu_baseline": cpu,
         "parity": parity,
     }
     print(json.dumps(out), flush=True)
 
 
-def kernel_roofline(op, f, jbn, work, n, reps):
-    """The zeta-direction pass of the fused flux-divergence of a momentum
-    equation is the largest single kernel of the stage (it reads the most
-    fields).  Here the bandwidth figure is taken on the solver's core unit,
-    one compact derivative along zeta of one field: algorithmic bytes =
-    one FP64 read + one FP64 write per point = 16 B/pt."""
-    import torch
-    src = f["u"][0]
-    out = work["amut"].clone()
-    stream = torch.cuda.current_stream()
-    for _ in range(3):
-        op.derivative(src, 2, out=out)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        op.derivative(src, 2, out=out)
-    e1.record(stream)
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    bytes_launch = 16.0 * n ** 3
-    gbs = bytes_launch / (ms * 1e-3) / 1e9
-    return {"kernel": "k_lines_strided (compact derivative along zeta, 1 field)",
-            "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": gbs / HBM_PEAK_GBS, "traffic": None,
-            "algorithmic_bytes_per_launch": bytes_launch, "avg_launch_ms": ms}
+# HBM bytes per launch of kernels measured with rocprofv3 --pmc (FETCH_SIZE
+# and WRITE_SIZE, scaled by the same counters' reading for a copy kernel of
+# known size); the profile files under profiles/ hold the runs.
+MEASURED_TRAFFIC = {}
+
+
+def kernel_roofline(kernels, steps):
+    """Roofline of the kernel with the largest total time in the timed
+    region, from the library's per-launch events; plus the per-kernel
+    breakdown."""
+    breakdown = []
+    for k in kernels:
+        avg_ms = k["ms"] / k["launches"]
+        b = k["bytes"] / k["launches"]
+        name = k["label"] + (f" dir{k['dir']}" if k["dir"] >= 0 else "")
+        breakdown.append({"kernel": name, "launches_per_step": k["launches"] // steps,
+                          "us_per_launch": avg_ms * 1e3, "bytes_per_launch": b,
+                          "GBs": b / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0,
+                          "ms_per_step": k["ms"] / steps})
+    top = breakdown[0]
+    roof = {"kernel": top["kernel"], "bound": "hbm", "achieved": top["GBs"],
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": top["GBs"] / HBM_PEAK_GBS,
+            "traffic": MEASURED_TRAFFIC.get(top["kernel"]),
+            "algorithmic_bytes_per_launch": top["bytes_per_launch"],
+            "avg_launch_ms": top["us_per_launch"] * 1e-3,
+            "share_of_step": top["ms_per_step"] / sum(b["ms_per_step"] for b in breakdown)}
+    return roof, breakdown
 
 
 def cpu_baseline(f, jbn, sgd, c4, work, n, scheme):

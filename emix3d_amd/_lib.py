@@ -39,6 +39,14 @@ SIGNATURES = {
     "emx_divergence": ([_P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div": ([_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div_all": ([_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
+    "emx_dynsmag": ([_P, _P, _P, _P, _P, _P], _I),
+    "emx_mlp_nut": ([_P, _P, _P, _P, _P, _P, _P], _I),
+    "emx_mlp_gather": ([_P, _P, _P, _P, _P, _I, _P], _I),
+    "emx_mlp_forward_f32": ([_P, _P, _P, _I, _P, _P, _P], _I),
+    "emx_mlp_backward_work": ([_I], ctypes.c_size_t),
+    "emx_mlp_backward_f32": ([_P, _P, _P, _P, _P, _I, _P, _P, ctypes.c_size_t, _P], _I),
+    "emx_adam_f32": ([_P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                      ctypes.c_float, _I, _P], _I),
     "emx_deriv_adj": ([_P, _I, _P, _P, _D], _I),
     "emx_grad_adj": ([_P, _P, _P, _P, _P, _D, _D, _D], _I),
     "emx_divergence_adj": ([_P, _P, _P, _P, _D, _D, _D, _P], _I),

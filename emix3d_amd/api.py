@@ -19,6 +19,7 @@ from . import _lib
 SCHEMES = {"c4": 0, "c6f": 1, "c6c": 2}
 KIND_WAVE, KIND_STRIDED, KIND_STRIDED32, KIND_STRIDED16 = 0, 1, 2, 3
 PLAN_DERIV, PLAN_FLUX = 0, 1
+EMX_MLP_NPARAM, EMX_MLP_NNORM, EMX_MLP_ACT = 9089, 21, 193
 
 
 def _ptr(t):
@@ -33,6 +34,52 @@ def _ptr(t):
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _fptr(t, numel=None):
+    """Pointer of a contiguous float32 GPU tensor (of numel elements)."""
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float32:
+        raise TypeError("expected a float32 torch tensor")
+    if not t.is_cuda or not t.is_contiguous():
+        raise ValueError("expected a contiguous GPU tensor")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"expected {numel} elements, got {t.numel()}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _cur_stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def mlp_forward_f32(params, norm, feat):
+    """FP32 forward of the closure network on feature rows [n][10]:
+    (pre-activations [n][193], nu [n])."""
+    n = feat.shape[0]
+    act = torch.empty((n, EMX_MLP_ACT), dtype=torch.float32, device=feat.device)
+    nu = torch.empty(n, dtype=torch.float32, device=feat.device)
+    _lib.call("emx_mlp_forward_f32", _fptr(params, EMX_MLP_NPARAM), _fptr(norm, EMX_MLP_NNORM),
+              _fptr(feat, n * 10), n, _fptr(act), _fptr(nu), _cur_stream())
+    return act, nu
+
+
+def mlp_backward_f32(params, norm, feat, act, dnu, work=None):
+    """Parameter gradient sum_p dnu_p d nu_p / d params (FP32, 9089)."""
+    n = feat.shape[0]
+    need = int(_lib.lib().emx_mlp_backward_work(n))
+    if work is None or work.numel() < need:
+        work = torch.empty(need, dtype=torch.float32, device=feat.device)
+    grad = torch.empty(EMX_MLP_NPARAM, dtype=torch.float32, device=feat.device)
+    _lib.call("emx_mlp_backward_f32", _fptr(params, EMX_MLP_NPARAM), _fptr(norm, EMX_MLP_NNORM),
+              _fptr(feat, n * 10), _fptr(act, n * EMX_MLP_ACT), _fptr(dnu, n), n, _fptr(grad),
+              _fptr(work), work.numel(), _cur_stream())
+    return grad, work
+
+
+def adam_f32(params, grad, m, v, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step on the closure parameters (in place)."""
+    n = params.numel()
+    _lib.call("emx_adam_f32", _fptr(params), _fptr(grad, n), _fptr(m, n), _fptr(v, n), n,
+              float(lr), float(beta1), float(beta2), float(eps), int(step), _cur_stream())
 
 
 class Operator:
@@ -211,6 +258,37 @@ class Operator:
                   self._check(jbn, 11, "jbn"), self._check(dvel, 9, "dvel"), *ih,
                   self._check(rs, 5, "rs"), self._check(rsv, 5, "rsv"))
         return rs, rsv
+
+    # ------------------------------------------------ eddy-viscosity closures
+    def dynsmag(self, u, dvel, jbn, delta, nut=None):
+        """Dynamic Smagorinsky nu_t (Germano-Lilly, box test filter of ratio
+        2); the target of the learned closure."""
+        nut = self._empty(1)[0] if nut is None else nut
+        self._stream()
+        _lib.call("emx_dynsmag", self._ctx, self._check(u, 3, "u"), self._check(dvel, 9, "dvel"),
+                  self._check(jbn, 11, "jbn"), self._check(delta), self._check(nut))
+        return nut
+
+    def mlp_nut(self, params, norm, dvel, jbn, delta, nut=None):
+        """nu_t of the learned closure at every point (BF16 matrix cores)."""
+        nut = self._empty(1)[0] if nut is None else nut
+        self._stream()
+        _lib.call("emx_mlp_nut", self._ctx, _fptr(params, EMX_MLP_NPARAM),
+                  _fptr(norm, EMX_MLP_NNORM), self._check(dvel, 9, "dvel"),
+                  self._check(jbn, 11, "jbn"), self._check(delta), self._check(nut))
+        return nut
+
+    def mlp_gather(self, dvel, jbn, delta, idx, feat=None):
+        """FP32 features [n][10] of the points idx (int64 linear indices)."""
+        n = idx.numel()
+        if idx.dtype != torch.int64 or not idx.is_contiguous():
+            raise ValueError("idx: contiguous int64")
+        feat = torch.empty((n, 10), dtype=torch.float32, device=self.device) if feat is None else feat
+        self._stream()
+        _lib.call("emx_mlp_gather", self._ctx, self._check(dvel, 9, "dvel"),
+                  self._check(jbn, 11, "jbn"), self._check(delta), ctypes.c_void_p(idx.data_ptr()),
+                  n, _fptr(feat, n * 10))
+        return feat
 
     # ------------------------------------------------------------- adjoints
     def derivative_adj(self, g, direction, inv_h=None, out=None):

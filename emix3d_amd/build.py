@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libemix3d_hip.so")
 
-SOURCES = ["lines.hip", "ops_deriv.hip", "ops_flux.hip", "ops_sgs.hip", "ops_adj.hip", "levelset.hip",
+SOURCES = ["lines.hip", "ops_deriv.hip", "ops_flux.hip", "ops_sgs.hip", "ops_adj.hip", "closure.hip", "levelset.hip",
            "capi.hip"]
 
 # -ffp-contract=off keeps the FP64 operation order of the reference (no

@@ -46,6 +46,7 @@ void free_ctx(emx_ctx* c) {
   if (!c) return;
   (void)hipFree(c->acc);
   (void)hipFree(c->sgs_tmp);
+  (void)hipFree(c->mlp_pack);
   (void)hipFree(c->ls_buf);
   (void)hipFree(c->stage);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -118,6 +119,7 @@ int emx_create(emx_ctx** out, int nx, int ny, int nz, int scheme,
   c->ls_bytes = ls_scratch_doubles(c->d) * sizeof(double);
   if (hipMalloc(&c->acc, 8 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->sgs_tmp, 4 * N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->mlp_pack, mlp_pack_elems() * 2) != hipSuccess ||
       hipMalloc(&c->ls_buf, c->ls_bytes) != hipSuccess) {
     (void)hipGetLastError();
     set_error("emx_create: device allocation of scratch for %zu points failed", N);
@@ -382,6 +384,44 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
   CHECK_PTR(rsv);
   const double ih[3] = {ihx, ihy, ihz};
   return op_flux_div_all(ctx, FluxPtrs{um, u, pres, vis, jbn, dvel}, ih, rs, rsv);
+}
+
+int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
+                const double* delta, double* nut) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(u);
+  CHECK_PTR(dvel);
+  CHECK_PTR(jbn);
+  CHECK_PTR(delta);
+  CHECK_PTR(nut);
+  return op_dynsmag(ctx, u, dvel, jbn, delta, nut);
+}
+
+int emx_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
+                const double* jbn, const double* delta, double* nut) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(params);
+  CHECK_PTR(norm);
+  CHECK_PTR(dvel);
+  CHECK_PTR(jbn);
+  CHECK_PTR(delta);
+  CHECK_PTR(nut);
+  return op_mlp_nut(ctx, params, norm, dvel, jbn, delta, nut);
+}
+
+int emx_mlp_gather(emx_ctx* ctx, const double* dvel, const double* jbn, const double* delta,
+                   const long long* idx, int n, float* feat) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(dvel);
+  CHECK_PTR(jbn);
+  CHECK_PTR(delta);
+  CHECK_PTR(idx);
+  CHECK_PTR(feat);
+  if (n <= 0) {
+    set_error("emx_mlp_gather: n = %d", n);
+    return EMX_ERR_ARG;
+  }
+  return op_mlp_gather(ctx, dvel, jbn, delta, idx, n, feat);
 }
 
 int emx_deriv_adj(emx_ctx* ctx, int dir, const double* g, double* out,

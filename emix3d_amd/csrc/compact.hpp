@@ -677,6 +677,7 @@ hipError_t launch_lines_s(const Prob& prob, const LineTab& T, const DirGeom& g,
           case 2: return launch_wave_c<S, 2>(prob, T, g, ih, st);
           case 4: return launch_wave_c<S, 4>(prob, T, g, ih, st);
           case 8: return launch_wave_c<S, 8>(prob, T, g, ih, st);
+          case 16: return launch_wave_c<S, 16>(prob, T, g, ih, st);
         }
         break;
       case KIND_STRIDED32:
@@ -820,16 +821,12 @@ inline int max_chunks(int kind) {
 // C that fits 64 lanes; eta/zeta lines 16 per wave, C=8 (lines up to 512
 // points), else 64 per wave with the smallest C giving <= 16 waves.  Flux
 // divergence: the same xi shape; eta/zeta 16 per wave with C=4 (the tile
-// form, lines up to 256 points), else C=8.
+// form, lines up to 256 points), else C=8 (up to 512).
 inline void default_plan(int op, int dir, int n, int* kind, int* C) {
   if (dir == 0) {
     *kind = KIND_WAVE;
     int c = 1;
     while (c * 64 < n) c *= 2;
-    if (op == PLAN_FLUX && c > 8) {  // long xi lines: the tile form
-      *kind = KIND_STRIDED16;
-      c = 8;
-    }
     *C = c;
   } else if (op == PLAN_FLUX) {
     *kind = KIND_STRIDED16;
@@ -849,7 +846,7 @@ inline void default_plan(int op, int dir, int n, int* kind, int* C) {
 inline bool valid_plan(int op, int kind, int C) {
   if (!valid_chunk(kind, C)) return false;
   if (op == PLAN_FLUX)
-    return kind == KIND_WAVE ? C <= 8 : (kind != KIND_STRIDED && C <= 8);
+    return kind == KIND_WAVE ? C <= 16 : (kind != KIND_STRIDED && C <= 8);
   return true;
 }
 

@@ -144,6 +144,7 @@ struct emx_ctx {
   // flux-divergence, four fields for the eddy-viscosity kernels.
   double* acc = nullptr;       // 8*N
   double* sgs_tmp = nullptr;   // 4*N
+  void* mlp_pack = nullptr;    // BF16 operands of the closure network
   // level-set scratch (allocated lazily on first level-set call)
   double* ls_buf = nullptr;
   size_t ls_bytes = 0;

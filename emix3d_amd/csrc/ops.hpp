@@ -32,6 +32,16 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
             const double* jbn, const double* sgd, double c4, double* amut,
             double* vis, double* vis5);
 
+int op_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
+               const double* delta, double* nut);
+
+// learned closure (closure.hip); packed operands in ctx->mlp_pack
+size_t mlp_pack_elems();
+int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
+               const double* jbn, const double* delta, double* nut);
+int op_mlp_gather(emx_ctx* ctx, const double* dvel, const double* jbn, const double* delta,
+                  const long long* idx, int n, float* feat);
+
 // adjoints (ops_adj.hip)
 int op_deriv_adj(emx_ctx* ctx, int dir, const double* g, double* out, double ih);
 int op_grad_adj(emx_ctx* ctx, const double* dvel_bar, const double* dtemp_bar,

@@ -184,4 +184,158 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
   return EMX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Dynamic Smagorinsky eddy viscosity (Germano et al. 1991 identity, Lilly
+// 1992 least squares).  The reference has no such model (its sgdm is the
+// structure-function model above); this is the target of the learned
+// closure.  Physical gradient on the diagonal-metric grid:
+// du_a/dx_b = m_b * dvel(3b+a) with m = jbn 1, 5, 9.  Test filter: the box
+// filter of ratio 2, weights (1/4, 1/2, 1/4) per direction with clamped
+// indices at the walls, summed x innermost, then y, then z (the separable
+// passes x, y, z in that order).
+
+__device__ __forceinline__ void sym_strain(const double* dvel, const double* jbn,
+                                           long long N, long long p, double S[6],
+                                           double* aS) {
+  const double m[3] = {jbn[p], jbn[4 * N + p], jbn[8 * N + p]};
+  double g[9];
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++) g[3 * a + b] = m[b] * dvel[(3 * b + a) * N + p];
+  // pairs (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+  double s2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    S[q] = 0.5 * (g[3 * ia[q] + ib[q]] + g[3 * ib[q] + ia[q]]);
+    s2 += (q < 3 ? 1.0 : 2.0) * S[q] * S[q];
+  }
+  *aS = sqrt(2.0 * s2);
+}
+
+// Box filter of NQ quantities formed per point by f(p, out[NQ]).
+template <int NQ, class F>
+__device__ __forceinline__ void box27(const Dims& d, int i, int j, int k, F f, double out[NQ]) {
+  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+  const int xs[3] = {i > 0 ? i - 1 : 0, i, i < d.nx - 1 ? i + 1 : d.nx - 1};
+  const int ys[3] = {j > 0 ? j - 1 : 0, j, j < d.ny - 1 ? j + 1 : d.ny - 1};
+  const int zs[3] = {k > 0 ? k - 1 : 0, k, k < d.nz - 1 ? k + 1 : d.nz - 1};
+  const double w[3] = {0.25, 0.5, 0.25};
+  double Z[NQ];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double Y[NQ];
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+      double v0[NQ], v1[NQ], v2[NQ];
+      const long long row = ys[b] * sy + zs[c] * sz;
+      f(row + xs[0], v0);
+      f(row + xs[1], v1);
+      f(row + xs[2], v2);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const double X = 0.25 * v0[q] + 0.5 * v1[q] + 0.25 * v2[q];
+        Y[q] = b == 0 ? w[0] * X : Y[q] + w[b] * X;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) Z[q] = c == 0 ? w[0] * Y[q] : Z[q] + w[c] * Y[q];
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) out[q] = Z[q];
+}
+
+// Germano-Lilly contractions at each point: with hat() the box filter,
+//   L_ij = hat(u_i u_j) - hat(u_i) hat(u_j),
+//   M_ij = 2 delta^2 (hat(|S| S_ij) - 4 |hat S| hat S_ij),
+//   LM = L:M, MM = M:M (off-diagonal pairs counted twice).
+__global__ void __launch_bounds__(256)
+k_dynsmag_lm(const double* u, const double* dvel, const double* jbn, const double* delta,
+             Dims d, double* LM, double* MM) {
+  const long long N = d.N();
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % d.nx), j = (int)((p / d.nx) % d.ny),
+              k = (int)(p / ((long long)d.nx * d.ny));
+    double hS[6], hSS[6], huu[6], hu[3];
+    box27<6>(d, i, j, k, [&](long long q, double* v) {
+      double a;
+      sym_strain(dvel, jbn, N, q, v, &a);
+    }, hS);
+    box27<6>(d, i, j, k, [&](long long q, double* v) {
+      double S[6], a;
+      sym_strain(dvel, jbn, N, q, S, &a);
+#pragma unroll
+      for (int t = 0; t < 6; t++) v[t] = a * S[t];
+    }, hSS);
+    box27<6>(d, i, j, k, [&](long long q, double* v) {
+#pragma unroll
+      for (int t = 0; t < 6; t++) v[t] = u[ia[t] * N + q] * u[ib[t] * N + q];
+    }, huu);
+    box27<3>(d, i, j, k, [&](long long q, double* v) {
+#pragma unroll
+      for (int t = 0; t < 3; t++) v[t] = u[t * N + q];
+    }, hu);
+    double s2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) s2 += (q < 3 ? 1.0 : 2.0) * hS[q] * hS[q];
+    const double ahS = sqrt(2.0 * s2);
+    const double d2 = delta[p] * delta[p];
+    double lm = 0.0, mm = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const double L = huu[q] - hu[ia[q]] * hu[ib[q]];
+      const double M = 2.0 * d2 * (hSS[q] - 4.0 * ahS * hS[q]);
+      const double wq = q < 3 ? 1.0 : 2.0;
+      lm += wq * L * M;
+      mm += wq * M * M;
+    }
+    LM[p] = lm;
+    MM[p] = mm;
+  }
+}
+
+// nu_t = max(0, C delta^2 |S|) with C = hat(LM) / hat(MM) (0 where
+// hat(MM) <= 0).
+__global__ void __launch_bounds__(256)
+k_dynsmag_nut(const double* LM, const double* MM, const double* dvel, const double* jbn,
+              const double* delta, Dims d, double* nut) {
+  const long long N = d.N();
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % d.nx), j = (int)((p / d.nx) % d.ny),
+              k = (int)(p / ((long long)d.nx * d.ny));
+    double h[2];
+    box27<2>(d, i, j, k, [&](long long q, double* v) {
+      v[0] = LM[q];
+      v[1] = MM[q];
+    }, h);
+    double S[6], aS;
+    sym_strain(dvel, jbn, N, p, S, &aS);
+    const double C = h[1] > 0.0 ? h[0] / h[1] : 0.0;
+    const double v = C * delta[p] * delta[p] * aS;
+    nut[p] = v > 0.0 ? v : 0.0;
+  }
+}
+
+int op_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
+               const double* delta, double* nut) {
+  const Dims d = ctx->d;
+  const long long N = d.N();
+  double* LM = ctx->sgs_tmp;
+  double* MM = ctx->sgs_tmp + N;
+  // reads u(3), dvel(9), jbn(3), delta; writes LM, MM / reads LM, MM,
+  // dvel(9), jbn(3), delta; writes nut
+  EMX_TIMED(ctx, "dynsmag_lm", 18.0 * 8.0 * N,
+            hipLaunchKernelGGL(k_dynsmag_lm, dim3(pointwise_blocks(N)), dim3(256), 0,
+                               ctx->stream, u, dvel, jbn, delta, d, LM, MM));
+  EMX_TIMED(ctx, "dynsmag_nut", 16.0 * 8.0 * N,
+            hipLaunchKernelGGL(k_dynsmag_nut, dim3(pointwise_blocks(N)), dim3(256), 0,
+                               ctx->stream, LM, MM, dvel, jbn, delta, d, nut));
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
 }  // namespace emx

@@ -74,7 +74,7 @@ int emx_set_stream(emx_ctx* ctx, void* stream);
  * 64, 32 or 16 neighbouring lines per wavefront with the line split into
  * chunks over the lanes and waves of a workgroup; C = points per chunk.
  * Valid for op 0: kind 0 C in {1,2,4,8,16}; kind 1 {8,16,32}; kind 2
- * {4,8,16}; kind 3 {4,8}.  Op 1: kind 0 C <= 8, kinds 2 and 3 C in {4,8}.
+ * {4,8,16}; kind 3 {4,8}.  Op 1: kind 0 C <= 16, kinds 2 and 3 C in {4,8}.
  * Used by the autotuner. */
 int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C);
 int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C);
@@ -153,6 +153,49 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
                      const double* pres, const double* vis, const double* jbn,
                      const double* dvel, double ihx, double ihy, double ihz,
                      double* rs, double* rsv);
+
+/* Dynamic Smagorinsky eddy viscosity (Germano identity, Lilly least
+ * squares, box test filter of ratio 2 with weights (1/4,1/2,1/4) per
+ * direction, clipped at 0).  No counterpart in the reference, whose model
+ * is the structure-function sgdm; it is the target of the learned closure.
+ * u: 3 comps; dvel: 9 comps (emx_grad layout); the physical gradient is
+ * du_a/dx_b = jbn(1+4b) * dvel(3b+a) (diagonal metrics, as the reference's
+ * hot path assumes); delta: filter width per point. */
+int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
+                const double* delta, double* nut);
+
+/* ---- learned closure -------------------------------------------------------
+ * No counterpart in the reference.  A 10 -> 64 -> 64 -> 64 -> 1 network
+ * (SiLU hidden, softplus output) maps the physical velocity gradient
+ * du_a/dx_b = jbn(1+4b) * dvel(3b+a) (feature 3a+b) and the filter width
+ * delta (feature 9) to nu_t >= 0.  params: FP32[EMX_MLP_NPARAM] =
+ * W1[64][10] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4;
+ * norm: FP32[EMX_MLP_NNORM] = in_shift[10] in_scale[10] out_scale, with
+ * x = (f - in_shift) * in_scale and nu_t = out_scale * softplus(z). */
+#define EMX_MLP_NPARAM 9089
+#define EMX_MLP_NNORM 21
+
+/* nu_t at every grid point; hidden layers on the matrix cores (BF16
+ * operands, FP32 accumulation), FP64 output. */
+int emx_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
+                const double* jbn, const double* delta, double* nut);
+
+/* Training.  Features of the n points idx (linear indices), FP32 [n][10]. */
+int emx_mlp_gather(emx_ctx* ctx, const double* dvel, const double* jbn, const double* delta,
+                   const long long* idx, int n, float* feat);
+/* FP32 forward of n feature rows: act [n][193] (pre-activations of the
+ * three hidden layers and the output), nu [n]. */
+int emx_mlp_forward_f32(const float* params, const float* norm, const float* feat, int n,
+                        float* act, float* nu, void* stream);
+/* Floats of work emx_mlp_backward_f32 needs for n rows. */
+size_t emx_mlp_backward_work(int n);
+/* grad[EMX_MLP_NPARAM] = sum over the n rows of dnu * d nu / d params. */
+int emx_mlp_backward_f32(const float* params, const float* norm, const float* feat,
+                         const float* act, const float* dnu, int n, float* grad, float* work,
+                         size_t work_floats, void* stream);
+/* One Adam step (bias-corrected; step counts from 1). */
+int emx_adam_f32(float* params, const float* grad, float* m, float* v, int n, float lr,
+                 float beta1, float beta2, float eps, int step, void* stream);
 
 /* ---- adjoints (transposes) ------------------------------------------------
  * No counterpart in the reference, which has no adjoint.  Each entry is the

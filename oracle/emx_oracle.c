@@ -1143,3 +1143,122 @@ int orc_flux_div_all_adj(int scheme, int nx, int ny, int nz, const double *rs_ba
     free(fb);
     return rc;
 }
+
+
+/* ---------------------------------------------------------------------- */
+/* Learned closure network (no reference counterpart).                    */
+/* ---------------------------------------------------------------------- */
+enum { MLP_H = 64, MLP_IN = 10 };
+#define MLP_W1 0
+#define MLP_B1 (MLP_W1 + MLP_H * MLP_IN)
+#define MLP_W2 (MLP_B1 + MLP_H)
+#define MLP_B2 (MLP_W2 + MLP_H * MLP_H)
+#define MLP_W3 (MLP_B2 + MLP_H)
+#define MLP_B3 (MLP_W3 + MLP_H * MLP_H)
+#define MLP_W4 (MLP_B3 + MLP_H)
+#define MLP_B4 (MLP_W4 + MLP_H)
+#define MLP_NPARAM (MLP_B4 + 1)
+
+static double silu(double z) { return z / (1.0 + exp(-z)); }
+static double dsilu(double z)
+{
+    double s = 1.0 / (1.0 + exp(-z));
+    return s * (1.0 + z * (1.0 - s));
+}
+static double softplus(double z) { return z > 30.0 ? z : log1p(exp(z)); }
+
+/* One point: pre-activations z1..z3 and activations h1..h3, returns z4. */
+static double mlp_point(const float *P, const float *norm, const double *f,
+                        double *z1, double *h1, double *z2, double *h2,
+                        double *z3, double *h3)
+{
+    double x[MLP_IN];
+    for (int k = 0; k < MLP_IN; k++) x[k] = (f[k] - norm[k]) * norm[MLP_IN + k];
+    for (int o = 0; o < MLP_H; o++) {
+        double s = P[MLP_B1 + o];
+        for (int k = 0; k < MLP_IN; k++) s += P[MLP_W1 + o * MLP_IN + k] * x[k];
+        z1[o] = s; h1[o] = silu(s);
+    }
+    for (int o = 0; o < MLP_H; o++) {
+        double s = P[MLP_B2 + o];
+        for (int k = 0; k < MLP_H; k++) s += P[MLP_W2 + o * MLP_H + k] * h1[k];
+        z2[o] = s; h2[o] = silu(s);
+    }
+    for (int o = 0; o < MLP_H; o++) {
+        double s = P[MLP_B3 + o];
+        for (int k = 0; k < MLP_H; k++) s += P[MLP_W3 + o * MLP_H + k] * h2[k];
+        z3[o] = s; h3[o] = silu(s);
+    }
+    double s = P[MLP_B4];
+    for (int k = 0; k < MLP_H; k++) s += P[MLP_W4 + k] * h3[k];
+    return s;
+}
+
+int orc_mlp_forward(const float *params, const float *norm, const double *feat,
+                    long n, double *nu)
+{
+    double z1[MLP_H], h1[MLP_H], z2[MLP_H], h2[MLP_H], z3[MLP_H], h3[MLP_H];
+    for (long p = 0; p < n; p++) {
+        double z4 = mlp_point(params, norm, feat + p * MLP_IN, z1, h1, z2, h2, z3, h3);
+        nu[p] = norm[2 * MLP_IN] * softplus(z4);
+    }
+    return 0;
+}
+
+int orc_mlp_grad(const float *params, const float *norm, const double *feat,
+                 const double *dnu, long n, double *grad)
+{
+    const float *P = params;
+    double z1[MLP_H], h1[MLP_H], z2[MLP_H], h2[MLP_H], z3[MLP_H], h3[MLP_H];
+    double d1[MLP_H], d2[MLP_H], d3[MLP_H], x[MLP_IN];
+    memset(grad, 0, sizeof(double) * MLP_NPARAM);
+    for (long p = 0; p < n; p++) {
+        const double *f = feat + p * MLP_IN;
+        for (int k = 0; k < MLP_IN; k++) x[k] = (f[k] - norm[k]) * norm[MLP_IN + k];
+        double z4 = mlp_point(P, norm, f, z1, h1, z2, h2, z3, h3);
+        /* d softplus / dz = sigmoid(z) */
+        double d4 = dnu[p] * norm[2 * MLP_IN] / (1.0 + exp(-z4));
+        grad[MLP_B4] += d4;
+        for (int k = 0; k < MLP_H; k++) {
+            grad[MLP_W4 + k] += d4 * h3[k];
+            d3[k] = d4 * P[MLP_W4 + k] * dsilu(z3[k]);
+        }
+        for (int o = 0; o < MLP_H; o++) {
+            grad[MLP_B3 + o] += d3[o];
+            for (int k = 0; k < MLP_H; k++) grad[MLP_W3 + o * MLP_H + k] += d3[o] * h2[k];
+        }
+        for (int k = 0; k < MLP_H; k++) {
+            double s = 0.0;
+            for (int o = 0; o < MLP_H; o++) s += P[MLP_W3 + o * MLP_H + k] * d3[o];
+            d2[k] = s * dsilu(z2[k]);
+        }
+        for (int o = 0; o < MLP_H; o++) {
+            grad[MLP_B2 + o] += d2[o];
+            for (int k = 0; k < MLP_H; k++) grad[MLP_W2 + o * MLP_H + k] += d2[o] * h1[k];
+        }
+        for (int k = 0; k < MLP_H; k++) {
+            double s = 0.0;
+            for (int o = 0; o < MLP_H; o++) s += P[MLP_W2 + o * MLP_H + k] * d2[o];
+            d1[k] = s * dsilu(z1[k]);
+        }
+        for (int o = 0; o < MLP_H; o++) {
+            grad[MLP_B1 + o] += d1[o];
+            for (int k = 0; k < MLP_IN; k++) grad[MLP_W1 + o * MLP_IN + k] += d1[o] * x[k];
+        }
+    }
+    return 0;
+}
+
+int orc_mlp_features(int nx, int ny, int nz, const double *dvel, const double *jbn,
+                     const double *delta, double *feat)
+{
+    long N = (long)nx * ny * nz;
+    for (long p = 0; p < N; p++) {
+        const double m[3] = {jbn[p], jbn[4 * N + p], jbn[8 * N + p]};
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                feat[p * MLP_IN + 3 * a + b] = m[b] * dvel[(3 * b + a) * N + p];
+        feat[p * MLP_IN + 9] = delta[p];
+    }
+    return 0;
+}

@@ -127,6 +127,23 @@ void orc_pad_mirror(int nx, int ny, int nz, int g, const double *in,
 int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gphys,
                 const double *delta, double *nut);
 
+/* ---- learned closure network (no reference counterpart) ------------------ */
+/* 10 -> 64 -> 64 -> 64 -> 1, SiLU hidden activations, softplus output.
+ * params (9089): W1[64][10] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]
+ * W4[64] b4; norm (21): in_shift[10] in_scale[10] out_scale.
+ * x_k = (f_k - in_shift_k) * in_scale_k; nu = out_scale * softplus(z4).
+ * Evaluated in FP64 from FP32 parameters.  Pinned by closed forms and
+ * central differences (tests/). */
+int orc_mlp_forward(const float *params, const float *norm, const double *feat,
+                    long n, double *nu);
+/* grad (9089, FP64) = sum_p dnu_p * d nu_p / d params. */
+int orc_mlp_grad(const float *params, const float *norm, const double *feat,
+                 const double *dnu, long n, double *grad);
+/* The 10 features of each point: du_a/dx_b = jbn(1+4b) * dvel(3b+a)
+ * (index 3a+b), then delta; feat is [N][10]. */
+int orc_mlp_features(int nx, int ny, int nz, const double *dvel, const double *jbn,
+                     const double *delta, double *feat);
+
 /* ---- adjoints ----------------------------------------------------------- */
 
 /* out = D^T g along dir, D = orc_deriv3d(scheme, dir, ., inv_h). */

@@ -61,6 +61,9 @@ def lib():
             "orc_pad_mirror": [_I, _I, _I, _I, _P, _P],
             "orc_weno5_derivs": [_I, _I, _I, _P, _P, _D, _D, _D, _P, _P],
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
+            "orc_mlp_forward": [_P, _P, _P, _L, _P],
+            "orc_mlp_grad": [_P, _P, _P, _P, _L, _P],
+            "orc_mlp_features": [_I, _I, _I, _P, _P, _P, _P],
             "orc_deriv3d_adj": [_I, _I, _I, _I, _I, _P, _D, _P],
             "orc_deriv_vel_adj": [_I, _I, _I, _I, _P, _P, _P, _P, _P],
             "orc_divergencia_adj": [_I, _I, _I, _I, _P, _P, _P, _D, _D, _D, _P],
@@ -164,6 +167,39 @@ def flux_div_all_adj(scheme, rs_bar, rsv_bar, um, u, vis, jbn, dvel, ih):
                                       _ptr(bars["vis_bar"]), _ptr(bars["dvel_bar"])),
            "flux_div_all_adj")
     return bars
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_P)
+
+
+def mlp_forward(params, norm, feat):
+    """FP64 network output for feature rows feat [n][10] (FP32 parameters)."""
+    (p, pp), (q, pq) = _f32(params), _f32(norm)
+    feat = np.ascontiguousarray(feat, dtype=np.float64)
+    nu = np.empty(feat.shape[0])
+    _check(lib().orc_mlp_forward(pp, pq, _ptr(feat), feat.shape[0], _ptr(nu)), "mlp_forward")
+    return nu
+
+
+def mlp_grad(params, norm, feat, dnu):
+    """sum_p dnu_p * d nu_p / d params (FP64, 9089 values)."""
+    (p, pp), (q, pq) = _f32(params), _f32(norm)
+    feat = np.ascontiguousarray(feat, dtype=np.float64)
+    dnu = np.ascontiguousarray(dnu, dtype=np.float64)
+    g = np.empty(9089)
+    _check(lib().orc_mlp_grad(pp, pq, _ptr(feat), _ptr(dnu), feat.shape[0], _ptr(g)), "mlp_grad")
+    return g
+
+
+def mlp_features(dvel, jbn, delta):
+    """[N][10] features: physical gradient du_a/dx_b (3a+b), delta."""
+    nx, ny, nz = dims(delta)
+    feat = np.empty((nx * ny * nz, 10))
+    _check(lib().orc_mlp_features(nx, ny, nz, _ptr(dvel), _ptr(jbn), _ptr(delta), _ptr(feat)),
+           "mlp_features")
+    return feat
 
 
 def jacobeano(scheme, x):
