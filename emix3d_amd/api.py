@@ -183,6 +183,17 @@ class Operator:
                   None if dtemp is None else self._check(dtemp, 3, "dtemp"), *ih)
         return dvel, dtemp
 
+    def grad_div(self, u, ih=None, jbn=None, dvel=None, div=None):
+        """Velocity gradient (9 comps) and the velocity divergence in
+        divergencia's conservative form, in one line pass per direction."""
+        ih = self.default_ih() if ih is None else ih
+        dvel = self._empty(9) if dvel is None else dvel
+        div = self._empty() if div is None else div
+        self._stream()
+        _lib.call("emx_grad_div", self._ctx, self._check(u, 3, "u"), self._check(jbn, 11, "jbn"),
+                  self._check(dvel, 9, "dvel"), self._check(div), *ih)
+        return dvel, div
+
     def jacobeano(self, x, jbn=None):
         """JACOBEANO: the 11 metric components from the coordinates x (3 comps)."""
         jbn = self._empty(11) if jbn is None else jbn
@@ -371,3 +382,58 @@ class Operator:
     def stage_buffers(self):
         return {"dvel": self._empty(9), "dtemp": self._empty(3), "amut": self._empty(),
                 "vis": self._empty(), "rs": self._empty(5), "rsv": self._empty(5)}
+
+
+class Batch:
+    """Many grids ("boxes") stepped together (emx_batch): per stage the
+    gradient fused with the velocity divergence and one WENO5 / SSP-RK3
+    level-set step for every box, one launch per phase over all boxes,
+    replayed from a HIP graph."""
+
+    def __init__(self, dims, scheme="c6c"):
+        if isinstance(scheme, str):
+            scheme = SCHEMES[scheme.lower()]
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        arr = (ctypes.c_int * (3 * len(self.dims)))(*[v for d in self.dims for v in d])
+        self._b = ctypes.c_void_p()
+        self._keep = {}
+        _lib.call("emx_batch_create", ctypes.byref(self._b), len(self.dims), arr, scheme,
+                  _cur_stream())
+
+    def close(self):
+        if self._b:
+            _lib.lib().emx_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def bind(self, i, u, jbn, phi, dvel, div, ih, dxyz, dt):
+        nx, ny, nz = self.dims[i]
+        for t, nc, name in ((u, 3, "u"), (jbn, 11, "jbn"), (phi, None, "phi"), (dvel, 9, "dvel"),
+                            (div, None, "div")):
+            want = (nz, ny, nx) if nc is None else (nc, nz, ny, nx)
+            if tuple(t.shape) != want:
+                raise ValueError(f"box {i} {name}: shape {tuple(t.shape)} != {want}")
+        self._keep[i] = (u, jbn, phi, dvel, div)  # the batch holds raw pointers
+        _lib.call("emx_batch_bind", self._b, i, _ptr(u), _ptr(jbn), _ptr(phi), _ptr(dvel),
+                  _ptr(div), *map(float, ih), *map(float, dxyz), float(dt))
+
+    def set_plan(self, direction, lines_per_wave, chunk):
+        _lib.call("emx_batch_set_plan", self._b, direction, lines_per_wave, chunk)
+
+    def get_plan(self, direction):
+        lg, c = ctypes.c_int(), ctypes.c_int()
+        _lib.call("emx_batch_get_plan", self._b, direction, ctypes.byref(lg), ctypes.byref(c))
+        return lg.value, c.value
+
+    def tune(self, reps=5):
+        _lib.call("emx_batch_set_stream", self._b, _cur_stream())
+        _lib.call("emx_batch_tune", self._b, int(reps))
+
+    def stage(self, graph=True):
+        _lib.call("emx_batch_set_stream", self._b, _cur_stream())
+        _lib.call("emx_batch_stage", self._b, 1 if graph else 0)

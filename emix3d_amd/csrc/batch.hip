@@ -1,0 +1,475 @@
+// Batched multi-grid stage: many independent grids ("boxes") of uneven
+// sizes advanced together.  One stage = the velocity gradient fused with
+// the velocity divergence (ProbGradDiv, three directional line passes) and
+// one WENO5 / SSP-RK3 level-set step (pad, three stages, unpad).  Every
+// phase is ONE launch over all boxes: workgroups are dealt out box by box
+// from per-phase prefix tables (blk_start), each box with its own line
+// tables, geometry and data.  The phase sequence (eight launches) is
+// captured once in a HIP graph and replayed.  The line-solver shape of
+// each direction (lines per wavefront LG and points per work-item C, i.e.
+// the split between the serial Thomas sweep inside a chunk and the
+// parallel carry combination across chunks) is chosen by timing the
+// candidates on the bound data (emx_batch_tune).
+//
+// The reference has no batched form; the per-box operators it batches are
+// DERIV_VEL (solver/deriv_vel.f90), divergencia (solver/divergencia.f90)
+// and cuAdvectLsJB (src/LevelSet/lsTransportEc.cu:201-418).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ops_common.hpp"
+
+namespace emx {
+
+struct BatchShape {
+  int lg, C;
+};
+
+// Shapes the batched line pass is built for (NV = 4 fields per point).
+static const BatchShape kBatchShapes[] = {{16, 4}, {16, 8}, {32, 4}, {32, 8}, {64, 8}, {64, 16}};
+
+static int lg_kind(int lg) {
+  return lg == 64 ? KIND_STRIDED : (lg == 32 ? KIND_STRIDED32 : KIND_STRIDED16);
+}
+
+template <int S, int C, int LG>
+static void* batched_kernel() {
+  return (void*)k_lines_batched<S, C, LG, ProbGradDiv>;
+}
+
+template <int S>
+static void* batched_kernel_s(int lg, int C) {
+  if (lg == 16) return C == 4 ? batched_kernel<S, 4, 16>() : batched_kernel<S, 8, 16>();
+  if (lg == 32) return C == 4 ? batched_kernel<S, 4, 32>() : batched_kernel<S, 8, 32>();
+  return C == 8 ? batched_kernel<S, 8, 64>() : batched_kernel<S, 16, 64>();
+}
+
+static void* batched_kernel_for(int scheme, int lg, int C) {
+  switch (scheme) {
+    case EMX_SCHEME_C4: return batched_kernel_s<EMX_SCHEME_C4>(lg, C);
+    case EMX_SCHEME_C6F: return batched_kernel_s<EMX_SCHEME_C6F>(lg, C);
+    default: return batched_kernel_s<EMX_SCHEME_C6C>(lg, C);
+  }
+}
+
+}  // namespace emx
+
+struct emx_batch {
+  int device = 0;
+  int scheme = EMX_SCHEME_C6C;
+  int nbox = 0;
+  std::vector<emx::Dims> dims;
+  hipStream_t user_stream = nullptr;
+  hipStream_t stream = nullptr;  // own stream: graph capture needs one
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  struct Box {
+    const double* u = nullptr;
+    const double* jbn = nullptr;
+    double* phi = nullptr;
+    double* dvel = nullptr;
+    double* div = nullptr;
+    double ih[3] = {0, 0, 0};
+    double dxyz[3] = {0, 0, 0};
+    double dt = 0.0;
+    bool bound = false;
+  };
+  std::vector<Box> box;
+  // scratch
+  double* acc = nullptr;        // sum of N
+  double* ls = nullptr;         // sum of 9 * NG
+  std::vector<long long> acc_off, ls_off;
+  // line passes, per direction
+  emx::BatchShape shape[3];
+  std::vector<emx::BatchLines<emx::ProbGradDiv>> lines_h[3];
+  emx::BatchLines<emx::ProbGradDiv>* lines_d[3] = {nullptr, nullptr, nullptr};
+  int* blk_d[3] = {nullptr, nullptr, nullptr};
+  int nblk[3] = {0, 0, 0};
+  int threads[3] = {0, 0, 0};
+  size_t lds[3] = {0, 0, 0};
+  // level-set phases
+  std::vector<emx::LsBatchGrid> ls_h;
+  emx::LsBatchGrid* ls_d = nullptr;
+  int* blk_pad_d = nullptr;
+  int* blk_int_d = nullptr;
+  const double** jbn_d = nullptr;
+  int nblk_pad = 0, nblk_int = 0;
+  // state
+  bool dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace emx {
+namespace {
+
+void drop_graph(emx_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+void free_batch(emx_batch* b) {
+  if (!b) return;
+  drop_graph(b);
+  (void)hipFree(b->acc);
+  (void)hipFree(b->ls);
+  for (int d = 0; d < 3; d++) {
+    (void)hipFree(b->lines_d[d]);
+    (void)hipFree(b->blk_d[d]);
+  }
+  (void)hipFree(b->ls_d);
+  (void)hipFree(b->blk_pad_d);
+  (void)hipFree(b->blk_int_d);
+  (void)hipFree(b->jbn_d);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int box_n(const Dims& d, int dir) { return dir == 0 ? d.nx : (dir == 1 ? d.ny : d.nz); }
+
+bool shape_fits(const emx_batch* b, int dir, BatchShape s) {
+  for (const Dims& d : b->dims) {
+    const int n = box_n(d, dir);
+    if ((n + s.C - 1) / s.C > 1024 / s.lg) return false;
+  }
+  return true;
+}
+
+// Tables, descriptors and launch geometry of direction dir under shape s.
+int plan_lines(emx_batch* b, int dir, BatchShape s) {
+  if (!shape_fits(b, dir, s)) {
+    set_error("batch: lines along direction %d do not fit %d lines x %d points per item",
+              dir, s.lg, s.C);
+    return EMX_ERR_SIZE;
+  }
+  auto& H = b->lines_h[dir];
+  H.assign(b->nbox, BatchLines<ProbGradDiv>{});
+  std::vector<int> blk(b->nbox + 1, 0);
+  int maxthreads = 64;
+  size_t maxlds = 0;
+  for (int i = 0; i < b->nbox; i++) {
+    const Dims& d = b->dims[i];
+    const DirGeom g = dir_geom(d, dir);
+    BatchLines<ProbGradDiv>& L = H[i];
+    EMX_TRY(get_line_tab(b->scheme, g.n, s.C, lg_kind(s.lg), &L.T));
+    L.nlanes = g.nlanes;
+    L.gx = (g.nlanes + s.lg - 1) / s.lg;
+    L.si = g.si;
+    L.so = g.so;
+    L.sl = g.sl;
+    const emx_batch::Box& B = b->box[i];
+    L.ih = B.ih[dir];
+    L.prob = ProbGradDiv{B.u, B.jbn, B.dvel, b->acc + b->acc_off[i], B.div, d.N(), dir};
+    blk[i + 1] = blk[i] + L.gx * g.nouter;
+    maxthreads = std::max(maxthreads, ((L.T.P * s.lg + 63) / 64) * 64);
+    const size_t lds = (size_t)(5 * L.T.n + 1 + 2 * L.T.P * L.T.KB +
+                                ProbGradDiv::NV * (1024 / s.lg) * s.lg) * sizeof(double);
+    maxlds = std::max(maxlds, lds);
+  }
+  b->shape[dir] = s;
+  b->nblk[dir] = blk[b->nbox];
+  b->threads[dir] = maxthreads;
+  b->lds[dir] = maxlds;
+  void* k = batched_kernel_for(b->scheme, s.lg, s.C);
+  if (maxlds > 65536)
+    EMX_TRY_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
+  EMX_TRY_HIP(hipMemcpy(b->lines_d[dir], H.data(), H.size() * sizeof(H[0]),
+                        hipMemcpyHostToDevice));
+  EMX_TRY_HIP(hipMemcpy(b->blk_d[dir], blk.data(), blk.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
+  drop_graph(b);
+  return EMX_OK;
+}
+
+int launch_pass(emx_batch* b, int dir) {
+  void* k = batched_kernel_for(b->scheme, b->shape[dir].lg, b->shape[dir].C);
+  const BatchLines<ProbGradDiv>* grids = b->lines_d[dir];
+  const int* blk = b->blk_d[dir];
+  int ngrid = b->nbox;
+  void* args[] = {(void*)&grids, (void*)&blk, (void*)&ngrid};
+  EMX_TRY_HIP(hipLaunchKernel(k, dim3(b->nblk[dir]), dim3(b->threads[dir]), args, b->lds[dir],
+                              b->stream));
+  return EMX_OK;
+}
+
+int launch_stage(emx_batch* b) {
+  for (int d = 0; d < 3; d++) EMX_TRY(launch_pass(b, d));
+  return op_batch_ls_step(b->stream, b->ls_d, b->blk_pad_d, b->nblk_pad, b->blk_int_d,
+                          b->nblk_int, b->nbox);
+}
+
+// Descriptors of every phase from the bound boxes; pads the metrics.
+int commit(emx_batch* b) {
+  for (int i = 0; i < b->nbox; i++)
+    if (!b->box[i].bound) {
+      set_error("batch: box %d has no bound data", i);
+      return EMX_ERR_ARG;
+    }
+  for (int d = 0; d < 3; d++) EMX_TRY(plan_lines(b, d, b->shape[d]));
+  std::vector<int> bp(b->nbox + 1, 0), bi(b->nbox + 1, 0);
+  std::vector<const double*> jb(b->nbox);
+  b->ls_h.resize(b->nbox);
+  for (int i = 0; i < b->nbox; i++) {
+    const Dims& d = b->dims[i];
+    const emx_batch::Box& B = b->box[i];
+    const long long NG = ls_padded_points(d.nx, d.ny, d.nz);
+    double* base = b->ls + b->ls_off[i];
+    LsBatchGrid& G = b->ls_h[i];
+    G.nx = d.nx;
+    G.ny = d.ny;
+    G.nz = d.nz;
+    G.phi = B.phi;
+    G.vel = B.u;
+    G.P = base;
+    G.V = base + NG;
+    G.J = base + 4 * NG;
+    G.T1 = base + 7 * NG;
+    G.T2 = base + 8 * NG;
+    G.iX = 1.0 / B.dxyz[0];
+    G.iY = 1.0 / B.dxyz[1];
+    G.iZ = 1.0 / B.dxyz[2];
+    G.dt = B.dt;
+    bp[i + 1] = bp[i] + (int)((NG + 255) / 256);
+    bi[i + 1] = bi[i] + (int)((d.N() + 255) / 256);
+    jb[i] = B.jbn;
+  }
+  b->nblk_pad = bp[b->nbox];
+  b->nblk_int = bi[b->nbox];
+  EMX_TRY_HIP(hipMemcpy(b->ls_d, b->ls_h.data(), b->nbox * sizeof(LsBatchGrid),
+                        hipMemcpyHostToDevice));
+  EMX_TRY_HIP(hipMemcpy(b->blk_pad_d, bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
+  EMX_TRY_HIP(hipMemcpy(b->blk_int_d, bi.data(), bi.size() * sizeof(int), hipMemcpyHostToDevice));
+  EMX_TRY_HIP(hipMemcpy(b->jbn_d, jb.data(), jb.size() * sizeof(double*), hipMemcpyHostToDevice));
+  EMX_TRY(op_batch_ls_pad_metrics(b->stream, b->ls_d, b->blk_pad_d, b->nblk_pad, b->nbox,
+                                  b->jbn_d));
+  EMX_TRY_HIP(hipStreamSynchronize(b->stream));
+  b->dirty = false;
+  return EMX_OK;
+}
+
+}  // namespace
+}  // namespace emx
+
+using emx::set_error;
+
+extern "C" {
+
+int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme, void* stream) {
+  if (!out || !dims || nbox <= 0) {
+    set_error("emx_batch_create: bad arguments");
+    return EMX_ERR_ARG;
+  }
+  *out = nullptr;
+  if (scheme < 0 || scheme > 2) {
+    set_error("emx_batch_create: unknown scheme %d", scheme);
+    return EMX_ERR_ARG;
+  }
+  auto* b = new emx_batch();
+  b->scheme = scheme;
+  b->nbox = nbox;
+  b->user_stream = (hipStream_t)stream;
+  long long accN = 0, lsN = 0;
+  for (int i = 0; i < nbox; i++) {
+    const emx::Dims d{dims[3 * i], dims[3 * i + 1], dims[3 * i + 2]};
+    const int nmin = emx::scheme_min_n(scheme);
+    if (d.nx < nmin || d.ny < nmin || d.nz < nmin || d.nx > 512 || d.ny > 512 || d.nz > 512) {
+      set_error("emx_batch_create: box %d is %d x %d x %d (each side %d..512)", i, d.nx, d.ny,
+                d.nz, nmin);
+      delete b;
+      return EMX_ERR_SIZE;
+    }
+    b->dims.push_back(d);
+    b->acc_off.push_back(accN);
+    b->ls_off.push_back(lsN);
+    accN += d.N();
+    lsN += 9 * emx::ls_padded_points(d.nx, d.ny, d.nz);
+  }
+  b->box.resize(nbox);
+  if (hipGetDevice(&b->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess) {
+    set_error("emx_batch_create: stream or event creation failed");
+    emx::free_batch(b);
+    return EMX_ERR_HIP;
+  }
+  bool ok = hipMalloc(&b->acc, accN * sizeof(double)) == hipSuccess &&
+            hipMalloc(&b->ls, lsN * sizeof(double)) == hipSuccess &&
+            hipMalloc(&b->ls_d, nbox * sizeof(emx::LsBatchGrid)) == hipSuccess &&
+            hipMalloc(&b->blk_pad_d, (nbox + 1) * sizeof(int)) == hipSuccess &&
+            hipMalloc(&b->blk_int_d, (nbox + 1) * sizeof(int)) == hipSuccess &&
+            hipMalloc(&b->jbn_d, nbox * sizeof(double*)) == hipSuccess;
+  for (int d = 0; d < 3 && ok; d++)
+    ok = hipMalloc(&b->lines_d[d], nbox * sizeof(emx::BatchLines<emx::ProbGradDiv>)) ==
+             hipSuccess &&
+         hipMalloc(&b->blk_d[d], (nbox + 1) * sizeof(int)) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    set_error("emx_batch_create: device allocation failed (%lld + %lld doubles)", accN, lsN);
+    emx::free_batch(b);
+    return EMX_ERR_NOMEM;
+  }
+  // default shape: 16 lines per wavefront, the shortest chunk that fits
+  for (int d = 0; d < 3; d++) {
+    b->shape[d] = emx::BatchShape{16, 4};
+    if (!emx::shape_fits(b, d, b->shape[d])) b->shape[d] = emx::BatchShape{16, 8};
+  }
+  *out = b;
+  return EMX_OK;
+}
+
+int emx_batch_destroy(emx_batch* b) {
+  emx::free_batch(b);
+  return EMX_OK;
+}
+
+int emx_batch_bind(emx_batch* b, int i, const double* u, const double* jbn, double* phi,
+                   double* dvel, double* div, double ihx, double ihy, double ihz, double dx,
+                   double dy, double dz, double dt) {
+  if (!b || i < 0 || i >= b->nbox) {
+    set_error("emx_batch_bind: bad batch or box index %d", i);
+    return EMX_ERR_ARG;
+  }
+  if (!u || !jbn || !phi || !dvel || !div) {
+    set_error("emx_batch_bind: null array for box %d", i);
+    return EMX_ERR_ARG;
+  }
+  emx_batch::Box& B = b->box[i];
+  B.u = u;
+  B.jbn = jbn;
+  B.phi = phi;
+  B.dvel = dvel;
+  B.div = div;
+  B.ih[0] = ihx;
+  B.ih[1] = ihy;
+  B.ih[2] = ihz;
+  B.dxyz[0] = dx;
+  B.dxyz[1] = dy;
+  B.dxyz[2] = dz;
+  B.dt = dt;
+  B.bound = true;
+  b->dirty = true;
+  emx::drop_graph(b);
+  return EMX_OK;
+}
+
+int emx_batch_set_plan(emx_batch* b, int dir, int lg, int C) {
+  if (!b || dir < 0 || dir > 2) {
+    set_error("emx_batch_set_plan: bad arguments");
+    return EMX_ERR_ARG;
+  }
+  bool known = false;
+  for (const auto& s : emx::kBatchShapes) known |= (s.lg == lg && s.C == C);
+  if (!known) {
+    set_error("emx_batch_set_plan: shape %d lines x %d points is not built", lg, C);
+    return EMX_ERR_ARG;
+  }
+  if (!emx::shape_fits(b, dir, emx::BatchShape{lg, C})) {
+    set_error("emx_batch_set_plan: lines along %d do not fit %d x %d", dir, lg, C);
+    return EMX_ERR_SIZE;
+  }
+  b->shape[dir] = emx::BatchShape{lg, C};
+  b->dirty = true;
+  emx::drop_graph(b);
+  return EMX_OK;
+}
+
+int emx_batch_get_plan(const emx_batch* b, int dir, int* lg, int* C) {
+  if (!b || dir < 0 || dir > 2 || !lg || !C) {
+    set_error("emx_batch_get_plan: bad arguments");
+    return EMX_ERR_ARG;
+  }
+  *lg = b->shape[dir].lg;
+  *C = b->shape[dir].C;
+  return EMX_OK;
+}
+
+int emx_batch_tune(emx_batch* b, int reps) {
+  if (!b) {
+    set_error("emx_batch_tune: null batch");
+    return EMX_ERR_ARG;
+  }
+  if (reps < 1) reps = 5;
+  if (b->dirty) EMX_TRY(emx::commit(b));
+  hipEvent_t e0, e1;
+  EMX_TRY_HIP(hipEventCreate(&e0));
+  EMX_TRY_HIP(hipEventCreate(&e1));
+  int rc = EMX_OK;
+  for (int d = 0; d < 3 && rc == EMX_OK; d++) {
+    float best = 1e30f;
+    emx::BatchShape pick = b->shape[d];
+    for (const auto& s : emx::kBatchShapes) {
+      if (!emx::shape_fits(b, d, s)) continue;
+      rc = emx::plan_lines(b, d, s);
+      if (rc != EMX_OK) break;
+      // passes 0..d so that the accumulator a pass reads is written
+      for (int w = 0; w < 2 && rc == EMX_OK; w++)
+        for (int q = 0; q <= d && rc == EMX_OK; q++) rc = emx::launch_pass(b, q);
+      if (rc != EMX_OK) break;
+      (void)hipEventRecord(e0, b->stream);
+      for (int r = 0; r < reps && rc == EMX_OK; r++) rc = emx::launch_pass(b, d);
+      (void)hipEventRecord(e1, b->stream);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        set_error("emx_batch_tune: timing failed");
+        rc = EMX_ERR_HIP;
+        break;
+      }
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        pick = s;
+      }
+    }
+    if (rc == EMX_OK) rc = emx::plan_lines(b, d, pick);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int emx_batch_stage(emx_batch* b, int use_graph) {
+  if (!b) {
+    set_error("emx_batch_stage: null batch");
+    return EMX_ERR_ARG;
+  }
+  if (b->dirty) EMX_TRY(emx::commit(b));
+  EMX_TRY_HIP(hipEventRecord(b->ev_in, b->user_stream));
+  EMX_TRY_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+  if (use_graph) {
+    if (!b->exec) {
+      EMX_TRY_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = emx::launch_stage(b);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc != EMX_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (rc == EMX_OK) set_error("emx_batch_stage: graph capture failed: %s", hipGetErrorString(e));
+        return rc != EMX_OK ? rc : EMX_ERR_HIP;
+      }
+      b->graph = g;
+      EMX_TRY_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    EMX_TRY_HIP(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    EMX_TRY(emx::launch_stage(b));
+  }
+  EMX_TRY_HIP(hipEventRecord(b->ev_out, b->stream));
+  EMX_TRY_HIP(hipStreamWaitEvent(b->user_stream, b->ev_out, 0));
+  return EMX_OK;
+}
+
+int emx_batch_set_stream(emx_batch* b, void* stream) {
+  if (!b) {
+    set_error("emx_batch_set_stream: null batch");
+    return EMX_ERR_ARG;
+  }
+  b->user_stream = (hipStream_t)stream;
+  return EMX_OK;
+}
+
+}  // extern "C"

@@ -255,6 +255,17 @@ int emx_grad(emx_ctx* ctx, const double* u, const double* temp, double* dvel,
   return op_grad(ctx, u, temp, dvel, dtemp, ih);
 }
 
+int emx_grad_div(emx_ctx* ctx, const double* u, const double* jbn, double* dvel, double* div,
+                 double ihx, double ihy, double ihz) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(u);
+  CHECK_PTR(jbn);
+  CHECK_PTR(dvel);
+  CHECK_PTR(div);
+  const double ih[3] = {ihx, ihy, ihz};
+  return op_grad_div(ctx, u, jbn, dvel, div, ih);
+}
+
 int emx_metrics(emx_ctx* ctx, const double* x, double* jbn) {
   EMX_TRY(check_ctx(ctx));
   CHECK_PTR(x);

@@ -221,10 +221,12 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
 // starts at i*si + o*so; point l at +l*sl.  The Thomas factors and the banded
 // chunk-carry weights (weights below 1e-24 dropped) are staged in LDS while
 // the data loads are in flight; carries travel through LDS.
+// The body takes the workgroup's line group bx and outer index by, so that
+// the batched kernel (many grids in one launch) can share it.
 template <int S, int C, int LG, class Prob>
-__global__ void __launch_bounds__(1024)
-k_lines_strided(Prob prob, LineTab T, int nlanes, long long si, long long so,
-                long long sl, double ih) {
+__device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineTab& T,
+                                                   int nlanes, long long si, long long so,
+                                                   long long sl, double ih, int bx, int by) {
   constexpr int NV = Prob::NV;
   constexpr int W = C + 6;
   constexpr int PMAX = 1024 / LG;  // most chunks a workgroup holds
@@ -242,11 +244,11 @@ k_lines_strided(Prob prob, LineTab T, int nlanes, long long si, long long so,
   const int tid = threadIdx.x;
   const int li = tid % LG;
   const int c = tid / LG;  // chunk index
-  const int i = blockIdx.x * LG + li;
+  const int i = bx * LG + li;
   const bool act = i < nlanes && c < P;
   const int ic = i < nlanes ? i : nlanes - 1;
   const int cc = c < P ? c : P - 1;
-  const long long base = (long long)ic * si + (long long)blockIdx.y * so;
+  const long long base = (long long)ic * si + (long long)by * so;
   const int s = c * C;
 
   // window loads, unconditional (clamped addresses), all in flight at once
@@ -345,6 +347,43 @@ k_lines_strided(Prob prob, LineTab T, int nlanes, long long si, long long so,
       prob.store(base + (long long)l * sl, v);
     }
   }
+}
+
+template <int S, int C, int LG, class Prob>
+__global__ void __launch_bounds__(1024)
+k_lines_strided(Prob prob, LineTab T, int nlanes, long long si, long long so,
+                long long sl, double ih) {
+  lines_strided_body<S, C, LG>(prob, T, nlanes, si, so, sl, ih, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// Many grids in one launch: workgroup b belongs to the grid g with
+// blk_start[g] <= b < blk_start[g+1] and solves that grid's line group
+// (b - blk_start[g]) % gx, outer index (b - blk_start[g]) / gx.  Each grid
+// has its own table (line length), geometry and problem data.
+template <class Prob>
+struct BatchLines {
+  Prob prob;
+  LineTab T;
+  int nlanes, gx;
+  long long si, so, sl;
+  double ih;
+};
+
+template <int S, int C, int LG, class Prob>
+__global__ void __launch_bounds__(1024)
+k_lines_batched(const BatchLines<Prob>* grids, const int* blk_start, int ngrid) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = ngrid;  // blk_start[lo] <= b < blk_start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (blk_start[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const BatchLines<Prob>& G = grids[lo];
+  const int local = b - blk_start[lo];
+  lines_strided_body<S, C, LG>(G.prob, G.T, G.nlanes, G.si, G.so, G.sl, G.ih, local % G.gx,
+                               local / G.gx);
 }
 
 // ---------------------------------------------------------------------------

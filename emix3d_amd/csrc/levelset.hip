@@ -174,15 +174,15 @@ __device__ __forceinline__ double bdiff(const double* phi, PDims g, int ax,
 // lsTools.cu:587-673 -> "left", PhiDevMinusParameterJB :676-770 ->
 // "right"), the upwind right-hand side advect_RS_WENO (:361-417) and the
 // stage update.  src is the stage input, P the step's initial phi.
-__global__ void k_weno_stage(int stage, const double* src, const double* P,
-                             const double* vel, const double* j1,
-                             const double* j5, const double* j9, PDims g,
-                             double iX, double iY, double iZ, double dt,
-                             double* out) {
+__device__ __forceinline__ void weno_stage_point(int stage, const double* src,
+                                                 const double* P, const double* vel,
+                                                 const double* j1, const double* j5,
+                                                 const double* j9, PDims g, double iX,
+                                                 double iY, double iZ, double dt,
+                                                 double* out, long long id) {
   const long long NG = g.N();
   const long long sxy = (long long)g.x * g.y;
-  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
-       id += (long long)gridDim.x * blockDim.x) {
+  {
     const int x = (int)(id % g.x), y = (int)((id / g.x) % g.y), z = (int)(id / sxy);
     // ---- left stencils (PhiDevPlusParameterJB), positions along each axis
     int lx[5] = {x - 2, x - 1, x, x + 1, x + 2};
@@ -279,6 +279,93 @@ __global__ void k_weno_stage(int stage, const double* src, const double* P,
   }
 }
 
+__global__ void k_weno_stage(int stage, const double* src, const double* P,
+                             const double* vel, const double* j1,
+                             const double* j5, const double* j9, PDims g,
+                             double iX, double iY, double iZ, double dt,
+                             double* out) {
+  const long long NG = g.N();
+  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
+       id += (long long)gridDim.x * blockDim.x)
+    weno_stage_point(stage, src, P, vel, j1, j5, j9, g, iX, iY, iZ, dt, out, id);
+}
+
+// ---------------------------------------------------------------------------
+// The same step for many grids in one launch per phase (the batched stage):
+// workgroup b serves grid gi with blk_start[gi] <= b < blk_start[gi+1],
+// points (b - blk_start[gi])*256 + threadIdx.x of that phase's index space
+// (padded points for pad and the stages, interior points for unpad).
+__device__ __forceinline__ int batch_grid(const int* blk_start, int ngrid) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = ngrid;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (blk_start[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256)
+k_batch_pad(const LsBatchGrid* grids, const int* blk_start, int ngrid) {
+  const int gi = batch_grid(blk_start, ngrid);
+  const LsBatchGrid& G = grids[gi];
+  const PDims g{G.nx + 2 * GC, G.ny + 2 * GC, G.nz + 2 * GC};
+  const long long p = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  if (p >= g.N()) return;
+  const int i = (int)(p % g.x), j = (int)((p / g.x) % g.y), k = (int)(p / ((long long)g.x * g.y));
+  const long long N = (long long)G.nx * G.ny * G.nz, NG = g.N();
+  const long long src = mirror(i, G.nx) + (long long)G.nx * (mirror(j, G.ny) +
+                                                             (long long)G.ny * mirror(k, G.nz));
+  G.P[p] = G.phi[src];
+#pragma unroll
+  for (int c = 0; c < 3; c++) G.V[c * NG + p] = G.vel[c * N + src];
+}
+
+__global__ void __launch_bounds__(256)
+k_batch_weno_stage(const LsBatchGrid* grids, const int* blk_start, int ngrid, int stage) {
+  const int gi = batch_grid(blk_start, ngrid);
+  const LsBatchGrid& G = grids[gi];
+  const PDims g{G.nx + 2 * GC, G.ny + 2 * GC, G.nz + 2 * GC};
+  const long long NG = g.N();
+  const long long id = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  if (id >= NG) return;
+  const double* src = stage == 0 ? G.P : (stage == 1 ? G.T1 : G.T2);
+  double* out = stage == 0 ? G.T1 : (stage == 1 ? G.T2 : G.P);
+  weno_stage_point(stage, src, G.P, G.V, G.J, G.J + NG, G.J + 2 * NG, g, G.iX, G.iY, G.iZ,
+                   G.dt, out, id);
+}
+
+__global__ void __launch_bounds__(256)
+k_batch_unpad(const LsBatchGrid* grids, const int* blk_start, int ngrid) {
+  const int gi = batch_grid(blk_start, ngrid);
+  const LsBatchGrid& G = grids[gi];
+  const long long p = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  const long long N = (long long)G.nx * G.ny * G.nz;
+  if (p >= N) return;
+  const int gx = G.nx + 2 * GC, gy = G.ny + 2 * GC;
+  const int i = (int)(p % G.nx), j = (int)((p / G.nx) % G.ny), k = (int)(p / ((long long)G.nx * G.ny));
+  G.phi[p] = G.P[(long long)(i + GC) + (long long)gx * ((long long)(j + GC) + (long long)gy * (k + GC))];
+}
+
+__global__ void __launch_bounds__(256)
+k_batch_pad_metrics(const LsBatchGrid* grids, const int* blk_start, int ngrid,
+                    const double* const* jbn) {
+  const int gi = batch_grid(blk_start, ngrid);
+  const LsBatchGrid& G = grids[gi];
+  const PDims g{G.nx + 2 * GC, G.ny + 2 * GC, G.nz + 2 * GC};
+  const long long p = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  if (p >= g.N()) return;
+  const int i = (int)(p % g.x), j = (int)((p / g.x) % g.y), k = (int)(p / ((long long)g.x * g.y));
+  const long long N = (long long)G.nx * G.ny * G.nz, NG = g.N();
+  const long long src = mirror(i, G.nx) + (long long)G.nx * (mirror(j, G.ny) +
+                                                             (long long)G.ny * mirror(k, G.nz));
+  const double* J = jbn[gi];
+  G.J[p] = J[src];
+  G.J[NG + p] = J[4 * N + src];
+  G.J[2 * NG + p] = J[8 * N + src];
+}
+
 unsigned blocks_for(long long n) {
   long long b = (n + 255) / 256;
   if (b > 131072) b = 131072;
@@ -290,6 +377,29 @@ PDims padded(const Dims& d) { return PDims{d.nx + 2 * GC, d.ny + 2 * GC, d.nz + 
 }  // namespace
 
 size_t ls_scratch_doubles(const Dims& d) { return (size_t)10 * padded(d).N(); }
+
+long long ls_padded_points(int nx, int ny, int nz) {
+  return (long long)(nx + 2 * GC) * (ny + 2 * GC) * (nz + 2 * GC);
+}
+
+int op_batch_ls_pad_metrics(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
+                            int nblk_pad, int ngrid, const double* const* jbn) {
+  hipLaunchKernelGGL(k_batch_pad_metrics, dim3(nblk_pad), dim3(256), 0, st, grids, blk_pad,
+                     ngrid, jbn);
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+int op_batch_ls_step(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
+                     int nblk_pad, const int* blk_int, int nblk_int, int ngrid) {
+  hipLaunchKernelGGL(k_batch_pad, dim3(nblk_pad), dim3(256), 0, st, grids, blk_pad, ngrid);
+  for (int stage = 0; stage < 3; stage++)
+    hipLaunchKernelGGL(k_batch_weno_stage, dim3(nblk_pad), dim3(256), 0, st, grids, blk_pad,
+                       ngrid, stage);
+  hipLaunchKernelGGL(k_batch_unpad, dim3(nblk_int), dim3(256), 0, st, grids, blk_int, ngrid);
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
 
 int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
                       const double* sobject, double dx, double dy, double dz,

@@ -18,6 +18,8 @@ const char* last_error();
 int op_deriv(emx_ctx* ctx, int dir, const double* f, double* df, double ih);
 int op_grad(emx_ctx* ctx, const double* u, const double* temp, double* dvel,
             double* dtemp, const double ih[3]);
+int op_grad_div(emx_ctx* ctx, const double* u, const double* jbn, double* dvel, double* div,
+                const double ih[3]);
 int op_flux(emx_ctx* ctx, int neq, const FluxPtrs& fp, double* flux);
 int op_divergence(emx_ctx* ctx, const double* flux, const double* jbn,
                   const double ih[3], double* rs, double* rsv);
@@ -52,6 +54,28 @@ int op_flux_div_all_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_ba
                         const FluxPtrs& fp, const double ih[3], double* um_bar,
                         double* u_bar, double* pres_bar, double* vis_bar,
                         double* dvel_bar);
+
+// One grid of the batched level-set step: dimensions, the caller's phi and
+// velocity (3 comps), and the batch's padded buffers: P (phi), V (velocity,
+// 3), J (jbn 1/5/9, padded once when bound), T1, T2 (RK stages).
+struct LsBatchGrid {
+  int nx, ny, nz;
+  double* phi;
+  const double* vel;
+  double* P;
+  double* V;
+  double* J;
+  double* T1;
+  double* T2;
+  double iX, iY, iZ, dt;
+};
+long long ls_padded_points(int nx, int ny, int nz);
+// blk_pad / blk_int: per-grid first workgroup (ngrid + 1 entries) over the
+// padded and the interior points, 256 points per workgroup.
+int op_batch_ls_pad_metrics(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
+                            int nblk_pad, int ngrid, const double* const* jbn);
+int op_batch_ls_step(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
+                     int nblk_pad, const int* blk_int, int nblk_int, int ngrid);
 
 // level set (levelset.hip); scratch is ctx->ls_buf
 size_t ls_scratch_doubles(const Dims& d);

@@ -33,6 +33,40 @@ struct ProbDeriv {
   }
 };
 
+// Velocity gradient (DERIV_VEL, solver/deriv_vel.f90:17-161) fused with the
+// velocity divergence in the projection of divergencia
+// (solver/divergencia.f90:225-354 with the velocity as the flux):
+// pass dir solves u, v, w and J*(m_dir*u_dir); dvel(3*dir+c) is written, the
+// fourth field accumulates and the last pass writes
+// div = (1/J)*((E_xi + F_eta) + G_zeta).
+struct ProbGradDiv {
+  static constexpr int NV = 4;
+  static const char* label() { return "grad_div"; }
+  double bytes_pt() const {
+    // u (3), J, metric; dvel (3); accumulator / 1/J and div
+    return 8.0 * (5 + 3 + (dir == 0 ? 1 : (dir == 1 ? 2 : 3)));
+  }
+  const double* u;    // 3 fields
+  const double* jbn;  // 11 fields
+  double* dvel;       // 9 fields
+  double* acc;
+  double* div;
+  long long N;
+  int dir;
+  __device__ __forceinline__ void load(long long p, double* v) const {
+#pragma unroll
+    for (int c = 0; c < 3; c++) v[c] = u[c * N + p];
+    v[3] = jbn[9 * N + p] * (jbn[(4 * dir) * N + p] * v[dir]);
+  }
+  __device__ __forceinline__ void store(long long p, const double* v) const {
+#pragma unroll
+    for (int c = 0; c < 3; c++) dvel[(3 * dir + c) * N + p] = v[c];
+    if (dir == 0) acc[p] = v[3];
+    else if (dir == 1) acc[p] = acc[p] + v[3];
+    else div[p] = jbn[10 * N + p] * (acc[p] + v[3]);
+  }
+};
+
 // Inputs of flujos (solver/flujos_compact.f90) addressed by linear index.
 struct FluxIn {
   const double* um;    // 5 comps

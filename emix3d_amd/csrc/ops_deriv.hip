@@ -102,4 +102,14 @@ int op_metrics(emx_ctx* ctx, const double* x, double* jbn, int* bad_dev) {
   return EMX_OK;
 }
 
+int op_grad_div(emx_ctx* ctx, const double* u, const double* jbn, double* dvel, double* div,
+                const double ih[3]) {
+  const long long N = ctx->d.N();
+  for (int d = 0; d < 3; d++) {
+    ProbGradDiv p{u, jbn, dvel, ctx->acc, div, N, d};
+    EMX_TRY(launch_lines(ctx, d, p, ih[d]));
+  }
+  return EMX_OK;
+}
+
 }  // namespace emx

@@ -160,3 +160,28 @@ def level_set_sphere_surface(x, seed, lengths=(1.0, 1.0, 1.0), radius_frac=0.25,
     d = torch.sqrt((x[0] - c[0]) ** 2 + (x[1] - c[1]) ** 2 + (x[2] - c[2]) ** 2) - radius_frac * L
     s = x[2] - surface_frac * lengths[2]
     return torch.minimum(d, s)
+
+
+def batch_dims(seed, nbox=48, lo=(32, 32, 32), hi=(192, 128, 96)):
+    """Box sizes of the batched multiphase stage: nbox sizes drawn uniformly
+    from {lo..hi} per axis (seeded), with the first box lo^3 and the second
+    the largest, hi."""
+    rng = np.random.default_rng(seed)
+    dims = [tuple(lo), tuple(hi)]
+    while len(dims) < nbox:
+        dims.append(tuple(int(rng.integers(lo[a], hi[a] + 1)) for a in range(3)))
+    return dims[:nbox]
+
+
+def batch_box(dims, seed, device="cpu"):
+    """One box of the batched stage on the unit cube: grid, the sine-mode
+    velocity, the sphere-plus-free-surface level set, the inverse spacings,
+    the spacings and the time step dt = 0.5 min(dx) / max|u| (CFL 0.5)."""
+    nx, ny, nz = dims
+    x = grid_uniform(nx, ny, nz, device=device)
+    u = velocity_sines(x, seed)
+    phi = level_set_sphere_surface(x, seed)
+    ih = (nx - 1.0, ny - 1.0, nz - 1.0)
+    dxyz = (1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    dt = 0.5 * min(dxyz) / float(u.abs().max())
+    return {"x": x, "u": u, "phi": phi, "ih": ih, "dxyz": dxyz, "dt": dt}

@@ -108,6 +108,13 @@ int emx_deriv(emx_ctx* ctx, int dir, const double* f, double* df,
 int emx_grad(emx_ctx* ctx, const double* u, const double* temp, double* dvel,
              double* dtemp, double ihx, double ihy, double ihz);
 
+/* Velocity gradient (as emx_grad, no temperature) fused with the velocity
+ * divergence in divergencia's conservative form with the velocity as the
+ * flux, div = (1/J)(D_xi(J xi_x u) + D_eta(J eta_y v) + D_zeta(J zeta_z w))
+ * (solver/divergencia.f90:225-354).  One line pass per direction. */
+int emx_grad_div(emx_ctx* ctx, const double* u, const double* jbn, double* dvel, double* div,
+                 double ihx, double ihy, double ihz);
+
 /* Grid metrics from coordinates x (3 comps): JACOBEANO
  * (solver/jacob_malla.f90:1-171), with the context's scheme and
  * ih = n-1 per direction.  Returns EMX_ERR_METRIC if any |J| <= 1e-20. */
@@ -163,6 +170,39 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
  * hot path assumes); delta: filter width per point. */
 int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
                 const double* delta, double* nut);
+
+/* ---- batched multi-grid stage ----------------------------------------------
+ * No counterpart in the reference.  Many independent grids ("boxes") of
+ * uneven sizes advanced together: per stage the gradient fused with the
+ * velocity divergence (emx_grad_div) and one WENO5 / SSP-RK3 level-set step
+ * (emx_ls_advect_weno5) for every box, each phase one launch over all
+ * boxes, the whole stage captured once in a HIP graph and replayed.  Each
+ * box side 5..512 points.  A batch runs on its own stream and orders itself
+ * after / before the caller's stream with events. */
+typedef struct emx_batch emx_batch;
+
+int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme,
+                     void* stream);
+int emx_batch_destroy(emx_batch* b);
+int emx_batch_set_stream(emx_batch* b, void* stream);
+/* Device arrays of box i (column-major, as the single-grid entries): u (3
+ * comps, in), jbn (11, in; 1/5/9 are padded once per bind), phi (in/out),
+ * dvel (9, out), div (out); inverse spacings of the derivatives; spacings
+ * and time step of the level-set step. */
+int emx_batch_bind(emx_batch* b, int i, const double* u, const double* jbn, double* phi,
+                   double* dvel, double* div, double ihx, double ihy, double ihz,
+                   double dx, double dy, double dz, double dt);
+/* Line-solver shape of direction dir: lg lines per wavefront (16, 32, 64)
+ * and C points per work-item; built pairs (16,4) (16,8) (32,4) (32,8)
+ * (64,8) (64,16). */
+int emx_batch_set_plan(emx_batch* b, int dir, int lg, int C);
+int emx_batch_get_plan(const emx_batch* b, int dir, int* lg, int* C);
+/* Autotuner: times every built shape of each direction's line pass on the
+ * bound data (reps launches each) and keeps the fastest. */
+int emx_batch_tune(emx_batch* b, int reps);
+/* One stage for all boxes; use_graph != 0 captures the launch sequence on
+ * the first call (after a bind or plan change) and replays it. */
+int emx_batch_stage(emx_batch* b, int use_graph);
 
 /* ---- learned closure -------------------------------------------------------
  * No counterpart in the reference.  A 10 -> 64 -> 64 -> 64 -> 1 network

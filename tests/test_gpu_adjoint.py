@@ -201,9 +201,11 @@ def test_autograd_gradcheck_tiny():
     u = f["u"].clone().requires_grad_(True)
     pres = f["pres"].clone().requires_grad_(True)
     vis = (1e-2 + 0 * f["pres"]).clone().requires_grad_(True)
+    # the operator is quadratic, so central differences are exact but for
+    # rounding, ~1e-16 |F| / eps with |F| up to ~1e4 here
     assert torch.autograd.gradcheck(lambda a, b, c, d: FluxDivOfState.apply(op, jbn, a, b, c, d),
-                                    (um, u, pres, vis), eps=1e-6, atol=1e-7, rtol=1e-6)
+                                    (um, u, pres, vis), eps=1e-3, atol=1e-6, rtol=1e-6)
     g = torch.randn((nz, ny, nx), dtype=torch.float64, device="cuda").requires_grad_(True)
     for d in range(3):
-        assert torch.autograd.gradcheck(lambda t: Derivative.apply(op, t, d), (g,), eps=1e-6,
-                                        atol=1e-7, rtol=1e-6)
+        assert torch.autograd.gradcheck(lambda t: Derivative.apply(op, t, d), (g,), eps=1e-3,
+                                        atol=1e-9, rtol=1e-7)
