@@ -47,6 +47,11 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--scheme", default="c6f")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only copy a 256 MiB buffer 5 times (the reference kernel for "
+                         "scaling hardware byte counters) and exit")
     return ap.parse_args()
 
 
@@ -61,6 +66,16 @@ def main():
     import torch
 
     from emix3d_amd import Operator, fields
+
+    if args.calibrate_copy:
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read_per_launch": src.numel() * 8,
+                          "bytes_written_per_launch": src.numel() * 8, "launches": 5}))
+        return
 
     n = args.n
     dev = torch.device("cuda:0")
@@ -103,6 +118,13 @@ def main():
     stage_bytes_pt = (22 + 24) * 8
     stage_gbs = stage_bytes_pt * pts / dt / 1e9
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_section(op, f, jbn, sgd, c4, work, args.steps, args.warmup)
+    batched = None
+    if not args.no_batched:
+        batched = batched_section(args.steps, args.warmup)
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -132,6 +154,8 @@ def main():
         "kernels": breakdown,
         "cpu_baseline": cpu,
         "parity": parity,
+        "batched": batched,
+        "learned": learned,
     }
     print(json.dumps(out), flush=True)
 
@@ -163,6 +187,136 @@ def kernel_roofline(kernels, steps):
             "avg_launch_ms": top["us_per_launch"] * 1e-3,
             "share_of_step": top["ms_per_step"] / sum(b["ms_per_step"] for b in breakdown)}
     return roof, breakdown
+
+
+def timed(fn, steps, warmup):
+    """Seconds per call of fn (whole-GPU work), synchronised on both sides."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def rel_l2(a, b):
+    import torch
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def learned_section(op, f, jbn, sgd, c4, work, steps, warmup):
+    """The 256^3 stage with the eddy viscosity from the learned closure
+    (BF16 matrix cores) against the same stage with dynamic Smagorinsky
+    computed directly: throughput of both, and the relative L2 error of
+    nu_t and of the viscous flux divergence rsv (equations 2..4) of the
+    learned form against the dynamic one.  The structure-function sgdm
+    (the reference's model) is scored against the same target."""
+    import torch
+    from emix3d_amd.closure import WEIGHTS, LearnedClosure
+    if not os.path.exists(WEIGHTS):
+        log("bench: no trained closure weights; learned section skipped")
+        return None
+    cl = LearnedClosure.load()
+    delta = sgd[0].contiguous()
+    u, um, temp, pres = f["u"], f["um"], f["temp"], f["pres"]
+    dv, dt_, vis, rs, rsv = work["dvel"], work["dtemp"], work["vis"], work["rs"], work["rsv"]
+    nut_d = torch.empty_like(delta)
+    nut_m = torch.empty_like(delta)
+
+    def stage(closure):
+        op.deriv_vel(u, temp, dvel=dv, dtemp=dt_)
+        if closure == "dyn":
+            op.dynsmag(u, dv, jbn, delta, nut=nut_d, c4=c4, vis=vis)
+        else:
+            cl.nut(op, dv, jbn, delta, out=nut_m, c4=c4, vis=vis)
+        op.flujos_divergencia_all(um, u, pres, vis, jbn, dv, rs=rs, rsv=rsv)
+
+    t_dyn = timed(lambda: stage("dyn"), steps, warmup)
+    rsv_dyn = rsv[1:4].clone()
+    t_mlp = timed(lambda: stage("mlp"), steps, warmup)
+    rsv_mlp = rsv[1:4].clone()
+    amut, _, _ = op.viscosidad_t(u, dv, jbn, sgd, c4, want_vis5=False)
+    op.flujos_divergencia_all(um, u, pres, (c4 + amut).contiguous(), jbn, dv, rs=rs, rsv=rsv)
+    rsv_sgdm = rsv[1:4].clone()
+    pts = float(delta.numel())
+    out = {"workload": "256^3 curvilinear stage (grad + closure + flux divergence of 5 eqs)",
+           "dynamic_smagorinsky_pts_per_s": pts / t_dyn,
+           "learned_pts_per_s": pts / t_mlp,
+           "speedup": t_dyn / t_mlp,
+           "nut_rel_l2_vs_dynamic": rel_l2(nut_m, nut_d),
+           "rsv_rel_l2_vs_dynamic": rel_l2(rsv_mlp, rsv_dyn),
+           "sgdm_nut_rel_l2_vs_dynamic": rel_l2(amut, nut_d),
+           "sgdm_rsv_rel_l2_vs_dynamic": rel_l2(rsv_sgdm, rsv_dyn),
+           "inference": "BF16 MFMA (16x16x32), FP32 accumulation, FP64 stage"}
+    log(f"bench: learned closure {out}")
+    return out
+
+
+def batched_section(steps, warmup):
+    """48 boxes of uneven sizes (32^3 .. 192x128x96), each with the sine-mode
+    velocity and a sphere-plus-free-surface level set: one stage = the
+    gradient fused with the velocity divergence plus one WENO5-RK3 level-set
+    step (CFL 0.5), for all boxes.  Batched (one launch per phase, HIP graph,
+    tuned line shapes) against the plain per-box form (the single-grid entry
+    points box after box), with the largest difference between the two."""
+    import torch
+    from emix3d_amd import Batch, Operator, fields
+    dims = fields.batch_dims(20261205, 48)
+    boxes, ops = [], []
+    b = Batch(dims, "c6c")
+    for i, d in enumerate(dims):
+        z = fields.batch_box(d, 1000 + i, device="cuda")
+        op = Operator(*d, "c6c")
+        z["jbn"] = op.jacobeano(z["x"])
+        del z["x"]
+        nx, ny, nz = d
+        z["phi0"] = z["phi"].clone()
+        for k in ("dvel", "dvel_p"):
+            z[k] = torch.empty((9, nz, ny, nx), dtype=torch.float64, device="cuda")
+        for k in ("div", "div_p"):
+            z[k] = torch.empty((nz, ny, nx), dtype=torch.float64, device="cuda")
+        z["phi_p"] = z["phi"].clone()
+        b.bind(i, z["u"], z["jbn"], z["phi"], z["dvel"], z["div"], z["ih"], z["dxyz"], z["dt"])
+        boxes.append(z)
+        ops.append(op)
+    pts = float(sum(nx * ny * nz for nx, ny, nz in dims))
+
+    def plain():
+        for z, op in zip(boxes, ops):
+            op.grad_div(z["u"], ih=z["ih"], jbn=z["jbn"], dvel=z["dvel_p"], div=z["div_p"])
+            op.advect_weno5(z["phi_p"], z["u"], z["jbn"], z["dxyz"], z["dt"])
+
+    b.tune(reps=3)
+    plans = [b.get_plan(d) for d in range(3)]
+    for z in boxes:
+        z["phi"].copy_(z["phi0"])
+        z["phi_p"].copy_(z["phi0"])
+    b.stage(graph=True)
+    plain()
+    torch.cuda.synchronize()
+    err = 0.0
+    for z in boxes:
+        for a, p in (("dvel", "dvel_p"), ("div", "div_p"), ("phi", "phi_p")):
+            err = max(err, float((z[a] - z[p]).abs().max() / z[p].abs().max()))
+    t_plain = timed(plain, steps, warmup)
+    t_batch = timed(lambda: b.stage(graph=True), steps, warmup)
+    t_eager = timed(lambda: b.stage(graph=False), steps, warmup)
+    bytes_pt = 240.0  # gradient 96 + divergence 72 + WENO5 step 72 B per point
+    out = {"workload": "48 boxes 32^3..192x128x96: grad + velocity divergence + WENO5-RK3 "
+                       "level-set step", "boxes": len(dims), "points": pts,
+           "plain_per_box_pts_per_s": pts / t_plain,
+           "batched_graph_pts_per_s": pts / t_batch,
+           "batched_eager_pts_per_s": pts / t_eager,
+           "speedup_vs_plain": t_plain / t_batch,
+           "batched_GBs_algorithmic": bytes_pt * pts / t_batch / 1e9,
+           "frac_of_hbm_peak": bytes_pt * pts / t_batch / 1e9 / HBM_PEAK_GBS,
+           "max_normwise_diff_vs_plain": err,
+           "line_shapes_lg_C": plans}
+    log(f"bench: batched {out}")
+    return out
 
 
 def cpu_baseline(f, jbn, sgd, c4, work, n, scheme):

@@ -39,7 +39,7 @@ SIGNATURES = {
     "emx_divergence": ([_P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div": ([_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div_all": ([_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
-    "emx_dynsmag": ([_P, _P, _P, _P, _P, _P], _I),
+    "emx_dynsmag": ([_P, _P, _P, _P, _P, _P, _D, _P], _I),
     "emx_grad_div": ([_P, _P, _P, _P, _P, _D, _D, _D], _I),
     "emx_batch_create": ([ctypes.POINTER(_P), _I, ctypes.POINTER(_I), _I, _P], _I),
     "emx_batch_destroy": ([_P], _I),
@@ -49,7 +49,7 @@ SIGNATURES = {
     "emx_batch_get_plan": ([_P, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "emx_batch_tune": ([_P, _I], _I),
     "emx_batch_stage": ([_P, _I], _I),
-    "emx_mlp_nut": ([_P, _P, _P, _P, _P, _P, _P], _I),
+    "emx_mlp_nut": ([_P, _P, _P, _P, _P, _P, _P, _D, _P], _I),
     "emx_mlp_gather": ([_P, _P, _P, _P, _P, _I, _P], _I),
     "emx_mlp_forward_f32": ([_P, _P, _P, _I, _P, _P, _P], _I),
     "emx_mlp_backward_work": ([_I], ctypes.c_size_t),

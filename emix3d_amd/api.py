@@ -271,22 +271,26 @@ class Operator:
         return rs, rsv
 
     # ------------------------------------------------ eddy-viscosity closures
-    def dynsmag(self, u, dvel, jbn, delta, nut=None):
+    def dynsmag(self, u, dvel, jbn, delta, nut=None, c4=0.0, vis=None):
         """Dynamic Smagorinsky nu_t (Germano-Lilly, box test filter of ratio
-        2); the target of the learned closure."""
+        2); the target of the learned closure.  With vis, also
+        vis = c4 + nu_t."""
         nut = self._empty(1)[0] if nut is None else nut
         self._stream()
         _lib.call("emx_dynsmag", self._ctx, self._check(u, 3, "u"), self._check(dvel, 9, "dvel"),
-                  self._check(jbn, 11, "jbn"), self._check(delta), self._check(nut))
+                  self._check(jbn, 11, "jbn"), self._check(delta), self._check(nut), float(c4),
+                  None if vis is None else self._check(vis))
         return nut
 
-    def mlp_nut(self, params, norm, dvel, jbn, delta, nut=None):
-        """nu_t of the learned closure at every point (BF16 matrix cores)."""
+    def mlp_nut(self, params, norm, dvel, jbn, delta, nut=None, c4=0.0, vis=None):
+        """nu_t of the learned closure at every point (BF16 matrix cores);
+        with vis, also vis = c4 + nu_t."""
         nut = self._empty(1)[0] if nut is None else nut
         self._stream()
         _lib.call("emx_mlp_nut", self._ctx, _fptr(params, EMX_MLP_NPARAM),
                   _fptr(norm, EMX_MLP_NNORM), self._check(dvel, 9, "dvel"),
-                  self._check(jbn, 11, "jbn"), self._check(delta), self._check(nut))
+                  self._check(jbn, 11, "jbn"), self._check(delta), self._check(nut), float(c4),
+                  None if vis is None else self._check(vis))
         return nut
 
     def mlp_gather(self, dvel, jbn, delta, idx, feat=None):

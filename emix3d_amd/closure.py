@@ -96,8 +96,8 @@ class LearnedClosure:
         np.savez(path, params=self.params.cpu().numpy(), norm=self.norm.cpu().numpy(),
                  **{k: np.asarray(v) for k, v in meta.items()})
 
-    def nut(self, op, dvel, jbn, delta, out=None):
-        return op.mlp_nut(self.params, self.norm, dvel, jbn, delta, nut=out)
+    def nut(self, op, dvel, jbn, delta, out=None, c4=0.0, vis=None):
+        return op.mlp_nut(self.params, self.norm, dvel, jbn, delta, nut=out, c4=c4, vis=vis)
 
     # ------------------------------------------------------------------ fit
     def fit_normalisation(self, fam, seeds, samples=65536, gen=None):

@@ -398,18 +398,18 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
 }
 
 int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
-                const double* delta, double* nut) {
+                const double* delta, double* nut, double c4, double* vis) {
   EMX_TRY(check_ctx(ctx));
   CHECK_PTR(u);
   CHECK_PTR(dvel);
   CHECK_PTR(jbn);
   CHECK_PTR(delta);
   CHECK_PTR(nut);
-  return op_dynsmag(ctx, u, dvel, jbn, delta, nut);
+  return op_dynsmag(ctx, u, dvel, jbn, delta, nut, c4, vis);
 }
 
 int emx_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
-                const double* jbn, const double* delta, double* nut) {
+                const double* jbn, const double* delta, double* nut, double c4, double* vis) {
   EMX_TRY(check_ctx(ctx));
   CHECK_PTR(params);
   CHECK_PTR(norm);
@@ -417,7 +417,7 @@ int emx_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const doub
   CHECK_PTR(jbn);
   CHECK_PTR(delta);
   CHECK_PTR(nut);
-  return op_mlp_nut(ctx, params, norm, dvel, jbn, delta, nut);
+  return op_mlp_nut(ctx, params, norm, dvel, jbn, delta, nut, c4, vis);
 }
 
 int emx_mlp_gather(emx_ctx* ctx, const double* dvel, const double* jbn, const double* delta,

@@ -86,7 +86,7 @@ __global__ void k_mlp_pack(const float* P, const float* norm, __bf16* A1, __bf16
 __global__ void __launch_bounds__(256)
 k_mlp_infer(const double* dvel, const double* jbn, const double* delta, long long N,
             const float* P, const float* norm, const __bf16* A1, const __bf16* A2,
-            const __bf16* A3, double* nut) {
+            const __bf16* A3, double* nut, double c4, double* vis) {
   const int l = threadIdx.x & 63;
   const int m = l % 16, g = l / 16;
   // weights and biases of this lane, kept in registers
@@ -174,7 +174,11 @@ k_mlp_infer(const double* dvel, const double* jbn, const double* delta, long lon
       for (int i = 0; i < 4; i++) z += w4[t][i] * h[t][i];
     z += __shfl_xor(z, 16);
     z += __shfl_xor(z, 32);
-    if (g == 0 && valid) nut[p] = (double)(oscale * softplus_f(z + b4));
+    if (g == 0 && valid) {
+      const double nu = (double)(oscale * softplus_f(z + b4));
+      nut[p] = nu;
+      if (vis) vis[p] = c4 * 1.0 + nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
+    }
   }
 }
 
@@ -362,7 +366,7 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, int n, floa
 size_t mlp_pack_elems() { return (size_t)(4 + 8 + 8) * 64 * 8; }
 
 int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
-               const double* jbn, const double* delta, double* nut) {
+               const double* jbn, const double* delta, double* nut, double c4, double* vis) {
   const long long N = ctx->d.N();
   __bf16* A1 = reinterpret_cast<__bf16*>(ctx->mlp_pack);
   __bf16* A2 = A1 + 4 * 64 * 8;
@@ -376,7 +380,7 @@ int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const doubl
   // reads dvel (9), jbn (3), delta; writes nut
   EMX_TIMED(ctx, "mlp_infer", 14.0 * 8.0 * N,
             hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                               dvel, jbn, delta, N, params, norm, A1, A2, A3, nut));
+                               dvel, jbn, delta, N, params, norm, A1, A2, A3, nut, c4, vis));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -35,12 +35,12 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
             double* vis, double* vis5);
 
 int op_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
-               const double* delta, double* nut);
+               const double* delta, double* nut, double c4, double* vis);
 
 // learned closure (closure.hip); packed operands in ctx->mlp_pack
 size_t mlp_pack_elems();
 int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
-               const double* jbn, const double* delta, double* nut);
+               const double* jbn, const double* delta, double* nut, double c4, double* vis);
 int op_mlp_gather(emx_ctx* ctx, const double* dvel, const double* jbn, const double* delta,
                   const long long* idx, int n, float* feat);
 

@@ -301,7 +301,7 @@ k_dynsmag_lm(const double* u, const double* dvel, const double* jbn, const doubl
 // hat(MM) <= 0).
 __global__ void __launch_bounds__(256)
 k_dynsmag_nut(const double* LM, const double* MM, const double* dvel, const double* jbn,
-              const double* delta, Dims d, double* nut) {
+              const double* delta, Dims d, double* nut, double c4, double* vis) {
   const long long N = d.N();
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
        p += (long long)gridDim.x * blockDim.x) {
@@ -316,12 +316,14 @@ k_dynsmag_nut(const double* LM, const double* MM, const double* dvel, const doub
     sym_strain(dvel, jbn, N, p, S, &aS);
     const double C = h[1] > 0.0 ? h[0] / h[1] : 0.0;
     const double v = C * delta[p] * delta[p] * aS;
-    nut[p] = v > 0.0 ? v : 0.0;
+    const double nu = v > 0.0 ? v : 0.0;
+    nut[p] = nu;
+    if (vis) vis[p] = c4 * 1.0 + nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
   }
 }
 
 int op_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
-               const double* delta, double* nut) {
+               const double* delta, double* nut, double c4, double* vis) {
   const Dims d = ctx->d;
   const long long N = d.N();
   double* LM = ctx->sgs_tmp;
@@ -333,7 +335,7 @@ int op_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* 
                                ctx->stream, u, dvel, jbn, delta, d, LM, MM));
   EMX_TIMED(ctx, "dynsmag_nut", 16.0 * 8.0 * N,
             hipLaunchKernelGGL(k_dynsmag_nut, dim3(pointwise_blocks(N)), dim3(256), 0,
-                               ctx->stream, LM, MM, dvel, jbn, delta, d, nut));
+                               ctx->stream, LM, MM, dvel, jbn, delta, d, nut, c4, vis));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -167,9 +167,10 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
  * is the structure-function sgdm; it is the target of the learned closure.
  * u: 3 comps; dvel: 9 comps (emx_grad layout); the physical gradient is
  * du_a/dx_b = jbn(1+4b) * dvel(3b+a) (diagonal metrics, as the reference's
- * hot path assumes); delta: filter width per point. */
+ * hot path assumes); delta: filter width per point.  When vis is not NULL
+ * it also receives vis = c4 + nu_t (viscosidad). */
 int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
-                const double* delta, double* nut);
+                const double* delta, double* nut, double c4, double* vis);
 
 /* ---- batched multi-grid stage ----------------------------------------------
  * No counterpart in the reference.  Many independent grids ("boxes") of
@@ -216,9 +217,10 @@ int emx_batch_stage(emx_batch* b, int use_graph);
 #define EMX_MLP_NNORM 21
 
 /* nu_t at every grid point; hidden layers on the matrix cores (BF16
- * operands, FP32 accumulation), FP64 output. */
+ * operands, FP32 accumulation), FP64 output; vis = c4 + nu_t when vis is
+ * not NULL. */
 int emx_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
-                const double* jbn, const double* delta, double* nut);
+                const double* jbn, const double* delta, double* nut, double c4, double* vis);
 
 /* Training.  Features of the n points idx (linear indices), FP32 [n][10]. */
 int emx_mlp_gather(emx_ctx* ctx, const double* dvel, const double* jbn, const double* delta,
