@@ -162,8 +162,9 @@ def main():
 
 # HBM bytes per launch of kernels measured with rocprofv3 --pmc (FETCH_SIZE
 # and WRITE_SIZE, scaled by the same counters' reading for a copy kernel of
-# known size); the profile files under profiles/ hold the runs.
-MEASURED_TRAFFIC = {}
+# known size, 1.9998 and 1.0); profiles/rk_stage_256_c6f_hbm_traffic.md
+# holds the runs.
+MEASURED_TRAFFIC = {"flux_div_all[8] dir0": 3303759521.0}
 
 
 def kernel_roofline(kernels, steps):

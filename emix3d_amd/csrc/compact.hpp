@@ -70,6 +70,23 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Line group of workgroup bxp (of gx along the lines) for LG lines per
+// wavefront.  A group of 8 doubles (64 bytes) shares each 128-byte cache
+// line with its neighbour.  Workgroups go to the eight XCDs round robin,
+// each XCD with its own L2, so the two halves of a line go to workgroups
+// whose ids differ by 8: the same XCD, dispatched together.
+template <int LG>
+__device__ __forceinline__ int line_group(int bxp, int gx) {
+  if constexpr (LG >= 16) {
+    return bxp;
+  } else {
+    const int base = bxp & ~15;
+    if (base + 16 > gx) return bxp;  // the last, incomplete set of 16
+    const int r = bxp & 15;
+    return base + 2 * (r & 7) + (r >> 3);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // One wavefront per line (xi lines, contiguous in memory).  Lines are
 // numbered L = j + nyl*k; base(L) = (L % nyl)*sj + (L / nyl)*sk, point l at
@@ -353,7 +370,8 @@ template <int S, int C, int LG, class Prob>
 __global__ void __launch_bounds__(1024)
 k_lines_strided(Prob prob, LineTab T, int nlanes, long long si, long long so,
                 long long sl, double ih) {
-  lines_strided_body<S, C, LG>(prob, T, nlanes, si, so, sl, ih, blockIdx.x, blockIdx.y);
+  lines_strided_body<S, C, LG>(prob, T, nlanes, si, so, sl, ih,
+                               line_group<LG>(blockIdx.x, gridDim.x), blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,8 +400,8 @@ k_lines_batched(const BatchLines<Prob>* grids, const int* blk_start, int ngrid) 
   }
   const BatchLines<Prob>& G = grids[lo];
   const int local = b - blk_start[lo];
-  lines_strided_body<S, C, LG>(G.prob, G.T, G.nlanes, G.si, G.so, G.sl, G.ih, local % G.gx,
-                               local / G.gx);
+  lines_strided_body<S, C, LG>(G.prob, G.T, G.nlanes, G.si, G.so, G.sl, G.ih,
+                               line_group<LG>(local % G.gx, G.gx), local / G.gx);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,7 +432,7 @@ k_lines_tile(Prob prob, LineTab T, int nlanes, long long si, long long so,
   const int tid = threadIdx.x;
   const int li = tid % LG;
   const int c = tid / LG;
-  const int i = blockIdx.x * LG + li;
+  const int i = line_group<LG>(blockIdx.x, gridDim.x) * LG + li;
   const bool act = i < nlanes && c < P;
   const int ic = i < nlanes ? i : nlanes - 1;
   const long long base = (long long)ic * si + (long long)blockIdx.y * so;
@@ -727,6 +745,10 @@ hipError_t launch_lines_s(const Prob& prob, const LineTab& T, const DirGeom& g,
         if (T.C == 4) return launch_strided_c<S, 4, 16>(prob, T, g, ih, st);
         if (T.C == 8) return launch_strided_c<S, 8, 16>(prob, T, g, ih, st);
         break;
+      case KIND_STRIDED8:
+        if (T.C == 2) return launch_strided_c<S, 2, 8>(prob, T, g, ih, st);
+        if (T.C == 4) return launch_strided_c<S, 4, 8>(prob, T, g, ih, st);
+        break;
     }
     return hipErrorInvalidValue;
   } else if constexpr (Prob::NV > 4) {
@@ -768,6 +790,12 @@ hipError_t launch_lines_s(const Prob& prob, const LineTab& T, const DirGeom& g,
         switch (T.C) {
           case 4: return launch_strided_c<S, 4, 16>(prob, T, g, ih, st);
           case 8: return launch_strided_c<S, 8, 16>(prob, T, g, ih, st);
+        }
+        break;
+      case KIND_STRIDED8:
+        switch (T.C) {
+          case 4: return launch_strided_c<S, 4, 8>(prob, T, g, ih, st);
+          case 8: return launch_strided_c<S, 8, 8>(prob, T, g, ih, st);
         }
         break;
     }
@@ -840,6 +868,7 @@ inline bool valid_chunk(int kind, int C) {
     case KIND_STRIDED: return C == 8 || C == 16 || C == 32;
     case KIND_STRIDED32: return C == 4 || C == 8 || C == 16;
     case KIND_STRIDED16: return C == 4 || C == 8;
+    case KIND_STRIDED8: return C == 2 || C == 4 || C == 8;
   }
   return false;
 }
@@ -852,6 +881,7 @@ inline int max_chunks(int kind) {
     case KIND_STRIDED: return 16;
     case KIND_STRIDED32: return 32;
     case KIND_STRIDED16: return 64;
+    case KIND_STRIDED8: return 128;
   }
   return 0;
 }
@@ -884,9 +914,12 @@ inline void default_plan(int op, int dir, int n, int* kind, int* C) {
 // Shapes each plan may take.
 inline bool valid_plan(int op, int kind, int C) {
   if (!valid_chunk(kind, C)) return false;
-  if (op == PLAN_FLUX)
-    return kind == KIND_WAVE ? C <= 16 : (kind != KIND_STRIDED && C <= 8);
-  return true;
+  if (op == PLAN_FLUX) {
+    if (kind == KIND_WAVE) return C <= 16;
+    if (kind == KIND_STRIDED8) return C <= 4;
+    return kind != KIND_STRIDED && C <= 8;
+  }
+  return kind != KIND_STRIDED8 || C >= 4;
 }
 
 }  // namespace emx

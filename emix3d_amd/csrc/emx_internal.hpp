@@ -43,6 +43,7 @@ enum LineKind {
   KIND_STRIDED = 1,    // 64 lines per wave, chunks over the waves
   KIND_STRIDED32 = 2,  // 32 lines per wave, two chunk slots per wave
   KIND_STRIDED16 = 3,  // 16 lines per wave, four chunk slots per wave
+  KIND_STRIDED8 = 4,   // 8 lines per wave, eight chunk slots per wave
 };
 
 // Device tables of one (scheme, n, chunk) combination.  All arrays live in

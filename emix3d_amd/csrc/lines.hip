@@ -108,7 +108,7 @@ int build_tab(int scheme_flags, int n, int C, int kind, LineTab* t) {
   }
   const int P = (n + C - 1) / C;
   const int pmax = kind == KIND_WAVE ? 64 : kind == KIND_STRIDED ? 16
-                 : kind == KIND_STRIDED32 ? 32 : 64;
+                 : kind == KIND_STRIDED32 ? 32 : kind == KIND_STRIDED8 ? 128 : 64;
   if (P > pmax) {
     set_error("line of %d points needs %d chunks of %d, more than the %d "
               "this launch shape (kind %d) holds", n, P, C, pmax, kind);

@@ -70,12 +70,12 @@ int emx_destroy(emx_ctx* ctx);
 int emx_set_stream(emx_ctx* ctx, void* stream);
 /* Line-solver launch shape of operation class op (0 = derivatives, gradient
  * and metrics; 1 = flux divergence) along direction dir (0=xi,1=eta,2=zeta):
- * kind 0 = one wavefront per line (lanes along the line); kinds 1, 2, 3 =
- * 64, 32 or 16 neighbouring lines per wavefront with the line split into
+ * kind 0 = one wavefront per line (lanes along the line); kinds 1, 2, 3, 4 =
+ * 64, 32, 16 or 8 neighbouring lines per wavefront with the line split into
  * chunks over the lanes and waves of a workgroup; C = points per chunk.
  * Valid for op 0: kind 0 C in {1,2,4,8,16}; kind 1 {8,16,32}; kind 2
- * {4,8,16}; kind 3 {4,8}.  Op 1: kind 0 C <= 16, kinds 2 and 3 C in {4,8}.
- * Used by the autotuner. */
+ * {4,8,16}; kind 3 {4,8}; kind 4 {4,8}.  Op 1: kind 0 C <= 16, kinds 2 and
+ * 3 C in {4,8}, kind 4 C in {2,4}.  Used by the autotuner. */
 int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C);
 int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C);
 

@@ -58,7 +58,7 @@ def test_derivative_all_directions(oracle, scheme, shape):
 
 
 PLANS = [(0, [1, 2, 4, 8, 16], 64), (1, [8, 16, 32], 16), (2, [4, 8, 16], 32),
-         (3, [4, 8], 64)]
+         (3, [4, 8], 64), (4, [4, 8], 128)]
 
 
 @pytest.mark.parametrize("kind,chunks,pmax", PLANS)
@@ -179,7 +179,7 @@ def test_flujos_divergencia(oracle, scheme):
 
 
 @pytest.mark.parametrize("scheme", SCHEMES)
-@pytest.mark.parametrize("plan", ["default", "strided", "wave_x_strided16"])
+@pytest.mark.parametrize("plan", ["default", "strided", "wave_x_strided16", "strided8"])
 def test_flux_div_all_equations(oracle, scheme, plan):
     """The RUNGEK equation loop in three passes equals flujos+divergencia of
     each equation, for every grouping the launch shapes select."""
@@ -195,6 +195,10 @@ def test_flux_div_all_equations(oracle, scheme, plan):
         op.set_plan(0, 0, 2, op=1)
         op.set_plan(1, 2, 8, op=1)
         op.set_plan(2, 3, 4, op=1)
+    elif plan == "strided8":
+        op.set_plan(0, 4, 4, op=1)
+        op.set_plan(1, 4, 2, op=1)
+        op.set_plan(2, 4, 4, op=1)
     ih = op.default_ih()
     D = {k: dev(z[k]) for k in ("um", "u", "pres", "jbn", "dvel")}
     rs, rsv = op.flujos_divergencia_all(D["um"], D["u"], D["pres"], dev(vis), D["jbn"],
