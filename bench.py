@@ -223,7 +223,10 @@ def learned_section(op, f, jbn, sgd, c4, work, steps, warmup):
     cl = LearnedClosure.load()
     delta = sgd[0].contiguous()
     u, um, temp, pres = f["u"], f["um"], f["temp"], f["pres"]
-    dv, dt_, vis, rs, rsv = work["dvel"], work["dtemp"], work["vis"], work["rs"], work["rsv"]
+    # own buffers: the stage's outputs stay as the plain stage left them
+    dv, dt_ = torch.empty_like(work["dvel"]), torch.empty_like(work["dtemp"])
+    vis, rs, rsv = (torch.empty_like(work["vis"]), torch.empty_like(work["rs"]),
+                    torch.empty_like(work["rsv"]))
     nut_d = torch.empty_like(delta)
     nut_m = torch.empty_like(delta)
 

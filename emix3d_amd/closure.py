@@ -50,12 +50,19 @@ def init_params(seed):
 
 
 class SnapshotFamily:
-    """Velocity snapshots (seeded sine modes) on one tanh-stretched grid,
-    with the grid's metrics, filter width (the sgdm width dxyzsgd) and the
-    structure-function model's geometry."""
+    """Velocity snapshots on one tanh-stretched grid, with the grid's
+    metrics, filter width (the sgdm width dxyzsgd) and the
+    structure-function model's geometry.  kind "sines": the sine-mode family
+    (8 modes per axis, seeded amplitudes and phases); "turbulent": the
+    solenoidal k^-5/3 field with the impeller jet.  seeds: the snapshot seeds
+    training cycles through; cache keeps generated snapshots on the GPU."""
 
-    def __init__(self, n, scheme="c6c", device="cuda"):
+    def __init__(self, n, scheme="c6c", device="cuda", kind="sines", seeds=range(1, 201),
+                 cache=False):
         self.n = n
+        self.kind = kind
+        self.seeds = list(seeds)
+        self.cache = {} if cache else None
         self.device = torch.device(device)
         self.op = Operator(n, n, n, scheme)
         self.x = fields.grid_tanh(n, n, n, device=self.device)
@@ -67,9 +74,16 @@ class SnapshotFamily:
         self.pres0 = zero
 
     def snapshot(self, seed):
-        u = fields.velocity_sines(self.x, seed).contiguous()
+        if self.cache is not None and seed in self.cache:
+            return self.cache[seed]
+        if self.kind == "sines":
+            u = fields.velocity_sines(self.x, seed).contiguous()
+        else:
+            u = fields.velocity_turbulent(self.x, seed).contiguous()
         dvel, _ = self.op.deriv_vel(u)
         nut = self.op.dynsmag(u, dvel, self.jbn, self.delta)
+        if self.cache is not None:
+            self.cache[seed] = (u, dvel, nut)
         return u, dvel, nut
 
     def rsv(self, u, dvel, nut):
@@ -100,14 +114,15 @@ class LearnedClosure:
         return op.mlp_nut(self.params, self.norm, dvel, jbn, delta, nut=out, c4=c4, vis=vis)
 
     # ------------------------------------------------------------------ fit
-    def fit_normalisation(self, fam, seeds, samples=65536, gen=None):
+    def fit_normalisation(self, fams, per_family=4, samples=65536, gen=None):
         feats, tg = [], []
-        N = fam.n ** 3
-        for s in seeds:
-            u, dvel, nut = fam.snapshot(s)
-            idx = torch.randint(0, N, (samples,), device=self.device, generator=gen)
-            feats.append(fam.op.mlp_gather(dvel, fam.jbn, fam.delta, idx))
-            tg.append(nut.reshape(-1)[idx])
+        for fam in fams:
+            N = fam.n ** 3
+            for s in fam.seeds[:per_family]:
+                u, dvel, nut = fam.snapshot(s)
+                idx = torch.randint(0, N, (samples,), device=self.device, generator=gen)
+                feats.append(fam.op.mlp_gather(dvel, fam.jbn, fam.delta, idx))
+                tg.append(nut.reshape(-1)[idx])
         f = torch.cat(feats).double()
         mean = f.mean(0)
         std = f.std(0).clamp_min(1e-30)
@@ -116,22 +131,27 @@ class LearnedClosure:
         assert norm.numel() == EMX_MLP_NNORM
         self.norm = norm.contiguous()
 
-    def train(self, fam, steps=600, batch=65536, lr=2e-3, flux_weight=0.5, nsnap=200,
-              seed=20261208, log=None):
-        """Adam on the two-term loss; snapshots seeds 1..nsnap in turn."""
+    def train(self, fams, steps=600, batch=65536, lr=2e-3, flux_weight=0.5, seed=20261208,
+              log=None):
+        """Adam on the two-term loss; the families take turns, each cycling
+        through its seeds."""
+        if isinstance(fams, SnapshotFamily):
+            fams = [fams]
         gen = torch.Generator(device=self.device).manual_seed(seed)
         if self.params is None:
             self.params = torch.as_tensor(init_params(seed), device=self.device).contiguous()
         if self.norm is None:
-            self.fit_normalisation(fam, range(1, 9), gen=gen)
+            self.fit_normalisation(fams, gen=gen)
         m = torch.zeros_like(self.params)
         v = torch.zeros_like(self.params)
-        N = fam.n ** 3
         work = None
         hist = []
         t0 = time.time()
         for step in range(steps):
-            u, dvel, target = fam.snapshot(1 + step % nsnap)
+            fam = fams[step % len(fams)]
+            k = step // len(fams)
+            u, dvel, target = fam.snapshot(fam.seeds[k % len(fam.seeds)])
+            N = fam.n ** 3
             idx = torch.randint(0, N, (batch,), device=self.device, generator=gen)
             feat = fam.op.mlp_gather(dvel, fam.jbn, fam.delta, idx)
             act, nu = mlp_forward_f32(self.params, self.norm, feat)
@@ -185,23 +205,41 @@ def evaluate(fam, closure, seeds):
     return {k: float(np.mean(v)) for k, v in out.items()}
 
 
+def training_families(device="cuda"):
+    """The training data: 200 snapshots of the sine-mode family at 128^3
+    (seeds 1..200), and the turbulent family the RK stage runs on, at 128^3
+    and 256^3 (seeds 1001.., kept on the GPU once generated), so that the
+    network sees the gradient statistics and filter widths it is used at."""
+    return [SnapshotFamily(128, kind="sines", seeds=range(1, 201), device=device),
+            SnapshotFamily(128, kind="turbulent", seeds=range(1001, 1017), cache=True,
+                           device=device),
+            SnapshotFamily(256, kind="turbulent", seeds=range(2001, 2007), cache=True,
+                           device=device)]
+
+
 def main():
     import argparse
     import sys
     ap = argparse.ArgumentParser(description="train the learned closure on the GPU")
-    ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--steps", type=int, default=1200)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--out", default=WEIGHTS)
     args = ap.parse_args()
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    fam = SnapshotFamily(args.n)
+    fams = training_families()
     cl = LearnedClosure()
-    hist = cl.train(fam, steps=args.steps, batch=args.batch, log=log)
-    ev = evaluate(fam, cl, range(201, 209))
-    log(f"held-out snapshots 201..208: {ev}")
-    cl.save(args.out, train_n=args.n, steps=args.steps, loss=np.asarray(hist),
-            eval_keys=np.asarray(list(ev.keys())), eval_vals=np.asarray(list(ev.values())))
+    hist = cl.train(fams, steps=args.steps, batch=args.batch, log=log)
+    evals = {}
+    held = {"sines128": (fams[0], range(201, 205)),
+            "turbulent128": (SnapshotFamily(128, kind="turbulent"), range(1101, 1103)),
+            "turbulent256": (SnapshotFamily(256, kind="turbulent"), range(2101, 2103))}
+    for name, (fam, seeds) in held.items():
+        ev = evaluate(fam, cl, seeds)
+        log(f"held-out {name} {list(seeds)}: {ev}")
+        for k, v in ev.items():
+            evals[f"{name}_{k}"] = v
+    cl.save(args.out, steps=args.steps, loss=np.asarray(hist),
+            eval_keys=np.asarray(list(evals.keys())), eval_vals=np.asarray(list(evals.values())))
     log(f"saved {args.out}")
 
 

@@ -118,7 +118,7 @@ int emx_create(emx_ctx** out, int nx, int ny, int nz, int scheme,
   const size_t N = (size_t)c->d.N();
   c->ls_bytes = ls_scratch_doubles(c->d) * sizeof(double);
   if (hipMalloc(&c->acc, 8 * N * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->sgs_tmp, 4 * N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->sgs_tmp, 2 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->mlp_pack, mlp_pack_elems() * 2) != hipSuccess ||
       hipMalloc(&c->ls_buf, c->ls_bytes) != hipSuccess) {
     (void)hipGetLastError();

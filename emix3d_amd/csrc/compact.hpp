@@ -889,8 +889,9 @@ inline int max_chunks(int kind) {
 // Default plans.  Derivatives: xi lines one wave per line with the smallest
 // C that fits 64 lanes; eta/zeta lines 16 per wave, C=8 (lines up to 512
 // points), else 64 per wave with the smallest C giving <= 16 waves.  Flux
-// divergence: the same xi shape; eta/zeta 16 per wave with C=4 (the tile
-// form, lines up to 256 points), else C=8 (up to 512).
+// divergence: the same xi shape; eta 16 per wave with C=4 (the tile form,
+// lines up to 256 points), zeta 32 per wave with C=8 (up to 256), else 16
+// per wave with C=8 (up to 512).
 inline void default_plan(int op, int dir, int n, int* kind, int* C) {
   if (dir == 0) {
     *kind = KIND_WAVE;
@@ -898,8 +899,10 @@ inline void default_plan(int op, int dir, int n, int* kind, int* C) {
     while (c * 64 < n) c *= 2;
     *C = c;
   } else if (op == PLAN_FLUX) {
-    *kind = KIND_STRIDED16;
-    *C = n <= 256 ? 4 : 8;
+    // zeta lines (the largest stride) measure faster with 32 lines per
+    // wavefront and C = 8 on MI355X
+    *kind = (dir == 2 && n <= 256) ? KIND_STRIDED32 : KIND_STRIDED16;
+    *C = (n <= 256 && dir != 2) ? 4 : 8;
   } else if (n <= 512) {
     *kind = KIND_STRIDED16;
     *C = 8;

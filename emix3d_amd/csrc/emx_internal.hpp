@@ -142,9 +142,9 @@ struct emx_ctx {
   // than the transposed solve supports)
   emx::LineTab adj[3] = {};
   // persistent scratch (device): eight accumulators for the fused
-  // flux-divergence, four fields for the eddy-viscosity kernels.
+  // flux-divergence, two fields for the dynamic eddy-viscosity model.
   double* acc = nullptr;       // 8*N
-  double* sgs_tmp = nullptr;   // 4*N
+  double* sgs_tmp = nullptr;   // 2*N (dynamic-model contractions, flags)
   void* mlp_pack = nullptr;    // BF16 operands of the closure network
   // level-set scratch (allocated lazily on first level-set call)
   double* ls_buf = nullptr;

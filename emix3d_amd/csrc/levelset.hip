@@ -27,14 +27,24 @@ __device__ __forceinline__ int mirror(int i, int n) {
   return s - GC;
 }
 
+// Workgroups go to the eight XCDs round robin (XCD = id mod 8), each XCD
+// with its own L2.  Renumbered by xcd_block, each XCD works through one
+// contiguous eighth of the index space, so the stencil neighbours of its
+// points (rows and planes away) sit in the same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned q = nb / 8;
+  return b < 8 * q ? (b % 8) * q + b / 8 : b;
+}
+
 // cuSwapToGhost + the three mirror passes, as one gather.
 __global__ void k_pad(const double* in, int nx, int ny, int nz, double* out,
                       PDims g) {
   const long long NG = g.N();
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < NG;
+  for (long long p = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; p < NG;
        p += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(p % g.x), j = (int)((p / g.x) % g.y),
-              k = (int)(p / ((long long)g.x * g.y));
+    const unsigned up = (unsigned)p, row = up / (unsigned)g.x;  // NG < 2^32
+    const int i = (int)(up - row * (unsigned)g.x), j = (int)(row % (unsigned)g.y),
+              k = (int)(row / (unsigned)g.y);
     const int is = mirror(i, nx), js = mirror(j, ny), ks = mirror(k, nz);
     out[p] = in[(long long)is + (long long)nx * ((long long)js + (long long)ny * ks)];
   }
@@ -44,10 +54,11 @@ __global__ void k_pad(const double* in, int nx, int ny, int nz, double* out,
 __global__ void k_unpad(const double* in, PDims g, int nx, int ny, int nz,
                         double* out) {
   const long long N = (long long)nx * ny * nz;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+  for (long long p = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; p < N;
        p += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(p % nx), j = (int)((p / nx) % ny),
-              k = (int)(p / ((long long)nx * ny));
+    const unsigned up = (unsigned)p, row = up / (unsigned)nx;
+    const int i = (int)(up - row * (unsigned)nx), j = (int)(row % (unsigned)ny),
+              k = (int)(row / (unsigned)ny);
     out[p] = in[(long long)(i + GC) + (long long)g.x * ((long long)(j + GC) + (long long)g.y * (k + GC))];
   }
 }
@@ -58,9 +69,11 @@ __global__ void k_ls_grad1(const double* S, const double* j1, const double* j5,
                            double iZ, double* dphi) {
   const long long NG = g.N();
   const long long sxy = (long long)g.x * g.y;
-  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
+  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
        id += (long long)gridDim.x * blockDim.x) {
-    const int x = (int)(id % g.x), y = (int)((id / g.x) % g.y), z = (int)(id / sxy);
+    const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
+    const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
+              z = (int)(row / (unsigned)g.y);
     long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x,
               kp = id + sxy, km = id - sxy;
     double factor = 0.5;
@@ -91,11 +104,13 @@ __global__ void k_ls_extrap_stage(int stage, const double* ext,
                                   double dt, double* out) {
   const long long NG = g.N();
   const long long sxy = (long long)g.x * g.y;
-  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
+  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
        id += (long long)gridDim.x * blockDim.x) {
     double rs = 0.0;
     if (S[id] > 0.0) {
-      const int x = (int)(id % g.x), y = (int)((id / g.x) % g.y), z = (int)(id / sxy);
+      const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
+      const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
+                z = (int)(row / (unsigned)g.y);
       long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x,
                 kp = id + sxy, km = id - sxy;
       long long id2 = id;
@@ -142,9 +157,10 @@ __device__ __forceinline__ double weno5(double b1, double b2, double b3,
   const double a1 = 0.1 / ((s1 + 1.0e-6) * (s1 + 1.0e-6));
   const double a2 = 0.6 / ((s2 + 1.0e-6) * (s2 + 1.0e-6));
   const double a3 = 0.3 / ((s3 + 1.0e-6) * (s3 + 1.0e-6));
-  const double o1 = a1 / (a1 + a2 + a3);
-  const double o2 = a2 / (a1 + a2 + a3);
-  const double o3 = a3 / (a1 + a2 + a3);
+  const double isum = 1.0 / (a1 + a2 + a3);  // one division for the three weights
+  const double o1 = a1 * isum;
+  const double o2 = a2 * isum;
+  const double o3 = a3 * isum;
   return ((o1 * (2.0 * b1 - 7.0 * b2 + 11.0 * b3) +
            o2 * (-1.0 * b2 + 5.0 * b3 + 2.0 * b4) +
            o3 * (2.0 * b3 + 5.0 * b4 - b5)) *
@@ -181,102 +197,79 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                                                  double iY, double iZ, double dt,
                                                  double* out, long long id) {
   const long long NG = g.N();
-  const long long sxy = (long long)g.x * g.y;
-  {
-    const int x = (int)(id % g.x), y = (int)((id / g.x) % g.y), z = (int)(id / sxy);
-    // ---- left stencils (PhiDevPlusParameterJB), positions along each axis
-    int lx[5] = {x - 2, x - 1, x, x + 1, x + 2};
-    int ly[5] = {y - 2, y - 1, y, y + 1, y + 2};
-    int lz[5] = {z - 2, z - 1, z, z + 1, z + 2};
-    // y-coordinate that id_jm1 ends up at (used by the z quirk)
-    int jm1y = y - 1;
-    if (x == 0) { lx[1] = x; lx[0] = x + 1; }
-    if (x == 1) { lx[0] = lx[1]; }
-    if (x == g.x - 1) { lx[3] = x; lx[4] = lx[1]; }
-    if (x == g.x - 2) { lx[4] = lx[3]; }
-    if (y == 0) { ly[1] = y; ly[0] = y + 1; jm1y = y; }
-    if (y == 1) { ly[0] = ly[1]; }
-    if (y == g.y - 1) { ly[3] = y; ly[4] = ly[1]; }
-    if (y == g.y - 2) { ly[4] = ly[3]; }
-    bool zquirk = false;
-    if (z == 0) { lz[1] = z; lz[0] = z + 1; }
-    if (z == 1) { lz[0] = lz[1]; }
-    if (z == g.z - 1) { lz[3] = z; zquirk = true; }  // id_kp2 = id_jm1
-    if (z == g.z - 2) { lz[4] = lz[3]; }
-    double left[3], right[3];
-    {
-      const double m = j1[id];
-      double b[5];
+  const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
+  const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
+            z = (int)(row / (unsigned)g.y);
+  const int pos[3] = {x, y, z};
+  const int ext[3] = {g.x, g.y, g.z};
+  const double ih[3] = {iX, iY, iZ};
+  const double met[3] = {j1[id], j5[id], j9[id]};
+  double r[3];
 #pragma unroll
-      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 0, lx[q], y, z, iX) * m;
-      left[0] = weno5(b[0], b[1], b[2], b[3], b[4]);
+  for (int c = 0; c < 3; c++) {
+    // upwinding (advect_RS_WENO, lsTools.cu:361-417) uses the left-biased
+    // derivative where the velocity is positive and the right-biased one
+    // where it is negative, so only that one is formed:
+    //   0.5(s+1) v L + 0.5|s-1| v R  =  v L (v > 0),  v R (v < 0),  0
+    const double v = vel[c * NG + id];
+    if (v == 0.0) {
+      r[c] = 0.0;
+      continue;
     }
-    {
-      const double m = j5[id];
-      double b[5];
+    const int q = pos[c], n = ext[c];
+    const double m = met[c];
+    const long long stride = c == 0 ? 1 : (c == 1 ? (long long)g.x : (long long)g.x * g.y);
+    // the backward differences (Dev1thO_Downwind, lsTools.cu:53-95) at
+    // offsets -2..+3 along the axis, each once; positions outside the
+    // padded grid are clamped and never used below
+    double bd[6];
 #pragma unroll
-      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 1, x, ly[q], z, iY) * m;
-      left[1] = weno5(b[0], b[1], b[2], b[3], b[4]);
+    for (int o = 0; o < 6; o++) {
+      int t = q - 2 + o;
+      t = t < 0 ? 0 : (t > n - 1 ? n - 1 : t);
+      const long long it = id + (long long)(t - q) * stride;
+      bd[o] = (t == 0 ? ih[c] * (src[it + stride] - src[it])
+                      : ih[c] * (src[it] - src[it - stride])) * m;
     }
-    {
-      const double m = j9[id];
-      double b[5];
+    double b[5];
+    if (v > 0.0) {
+      // left stencil (PhiDevPlusParameterJB, lsTools.cu:587-673): positions
+      // q-2..q+2 with the reference's substitutions at the edges
+      int l[5] = {q - 2, q - 1, q, q + 1, q + 2};
+      bool zquirk = false;
+      if (q == 0) { l[1] = q; l[0] = q + 1; }
+      if (q == 1) { l[0] = l[1]; }
+      if (c < 2) {
+        if (q == n - 1) { l[3] = q; l[4] = l[1]; }
+        if (q == n - 2) { l[4] = l[3]; }
+      } else {
+        if (q == n - 1) { l[3] = q; zquirk = true; }  // id_kp2 = id_jm1 (lsTools.cu:636)
+        if (q == n - 2) { l[4] = l[3]; }
+      }
 #pragma unroll
-      for (int q = 0; q < 4; q++) b[q] = bdiff(src, g, 2, x, y, lz[q], iZ) * m;
-      b[4] = zquirk ? bdiff(src, g, 2, x, jm1y, z, iZ) * m
-                    : bdiff(src, g, 2, x, y, lz[4], iZ) * m;
-      left[2] = weno5(b[0], b[1], b[2], b[3], b[4]);
-    }
-    // ---- right stencils (PhiDevMinusParameterJB): ip3, ip2, ip1, id, im1
-    int rx[5] = {x + 3, x + 2, x + 1, x, x - 1};
-    int ry[5] = {y + 3, y + 2, y + 1, y, y - 1};
-    int rz[5] = {z + 3, z + 2, z + 1, z, z - 1};
-    if (x == 0) { rx[4] = x; }
-    if (x == g.x - 1) { rx[2] = x; rx[1] = rx[4]; rx[0] = x - 2; }
-    if (x == g.x - 2) { rx[1] = rx[2]; rx[0] = x; }
-    if (x == g.x - 3) { rx[0] = rx[1]; }
-    if (y == 0) { ry[4] = y; }
-    if (y == g.y - 1) { ry[2] = y; ry[1] = ry[4]; ry[0] = y - 2; }
-    if (y == g.y - 2) { ry[1] = ry[2]; ry[0] = y; }
-    if (y == g.y - 3) { ry[0] = ry[1]; }
-    if (z == 0) { rz[4] = z; }
-    if (z == g.z - 1) { rz[2] = z; rz[1] = rz[4]; rz[0] = z - 2; }
-    if (z == g.z - 2) { rz[1] = rz[2]; rz[0] = z; }
-    if (z == g.z - 3) { rz[0] = rz[1]; }
-    {
-      const double m = j1[id];
-      double b[5];
+      for (int t = 0; t < 5; t++) b[t] = bd[l[t] - q + 2];
+      if (zquirk) {
+        const int jm1y = y == 0 ? y : y - 1;  // where id_jm1 ends up
+        b[4] = bdiff(src, g, 2, x, jm1y, z, iZ) * m;
+      }
+    } else {
+      // right stencil (PhiDevMinusParameterJB, lsTools.cu:676-770):
+      // positions q+3, q+2, q+1, q, q-1
+      int rr[5] = {q + 3, q + 2, q + 1, q, q - 1};
+      if (q == 0) { rr[4] = q; }
+      if (q == n - 1) { rr[2] = q; rr[1] = rr[4]; rr[0] = q - 2; }
+      if (q == n - 2) { rr[1] = rr[2]; rr[0] = q; }
+      if (q == n - 3) { rr[0] = rr[1]; }
 #pragma unroll
-      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 0, rx[q], y, z, iX) * m;
-      right[0] = weno5(b[0], b[1], b[2], b[3], b[4]);
+      for (int t = 0; t < 5; t++) b[t] = bd[rr[t] - q + 2];
     }
-    {
-      const double m = j5[id];
-      double b[5];
-#pragma unroll
-      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 1, x, ry[q], z, iY) * m;
-      right[1] = weno5(b[0], b[1], b[2], b[3], b[4]);
-    }
-    {
-      const double m = j9[id];
-      double b[5];
-#pragma unroll
-      for (int q = 0; q < 5; q++) b[q] = bdiff(src, g, 2, x, y, rz[q], iZ) * m;
-      right[2] = weno5(b[0], b[1], b[2], b[3], b[4]);
-    }
-    double r[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const double v = vel[c * NG + id];
-      const double s = (double)(v > 0.0) - (double)(v < 0.0);
-      r[c] = 0.5 * (s + 1.0) * v * left[c] + 0.5 * fabs(s - 1.0) * v * right[c];
-    }
-    const double rs = r[0] + r[1] + r[2];
-    const double sv = src[id];
-    if (stage == 0) out[id] = sv - dt * rs;
-    else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (sv - dt * rs);
-    else out[id] = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+    r[c] = v * weno5(b[0], b[1], b[2], b[3], b[4]);
   }
+  const double rs = r[0] + r[1] + r[2];
+  const double sv = src[id];
+  if (stage == 0) out[id] = sv - dt * rs;
+  else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (sv - dt * rs);
+  else out[id] = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
 }
 
 __global__ void k_weno_stage(int stage, const double* src, const double* P,
@@ -285,7 +278,7 @@ __global__ void k_weno_stage(int stage, const double* src, const double* P,
                              double iX, double iY, double iZ, double dt,
                              double* out) {
   const long long NG = g.N();
-  for (long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x; id < NG;
+  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
        id += (long long)gridDim.x * blockDim.x)
     weno_stage_point(stage, src, P, vel, j1, j5, j9, g, iX, iY, iZ, dt, out, id);
 }
@@ -296,7 +289,7 @@ __global__ void k_weno_stage(int stage, const double* src, const double* P,
 // points (b - blk_start[gi])*256 + threadIdx.x of that phase's index space
 // (padded points for pad and the stages, interior points for unpad).
 __device__ __forceinline__ int batch_grid(const int* blk_start, int ngrid) {
-  const int b = blockIdx.x;
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
   int lo = 0, hi = ngrid;
   while (hi - lo > 1) {
     const int mid = (lo + hi) >> 1;
@@ -311,9 +304,11 @@ k_batch_pad(const LsBatchGrid* grids, const int* blk_start, int ngrid) {
   const int gi = batch_grid(blk_start, ngrid);
   const LsBatchGrid& G = grids[gi];
   const PDims g{G.nx + 2 * GC, G.ny + 2 * GC, G.nz + 2 * GC};
-  const long long p = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  const long long p = (long long)((int)xcd_block(blockIdx.x, gridDim.x) - blk_start[gi]) * 256 + threadIdx.x;
   if (p >= g.N()) return;
-  const int i = (int)(p % g.x), j = (int)((p / g.x) % g.y), k = (int)(p / ((long long)g.x * g.y));
+  const unsigned up = (unsigned)p, row = up / (unsigned)g.x;
+  const int i = (int)(up - row * (unsigned)g.x), j = (int)(row % (unsigned)g.y),
+            k = (int)(row / (unsigned)g.y);
   const long long N = (long long)G.nx * G.ny * G.nz, NG = g.N();
   const long long src = mirror(i, G.nx) + (long long)G.nx * (mirror(j, G.ny) +
                                                              (long long)G.ny * mirror(k, G.nz));
@@ -328,7 +323,7 @@ k_batch_weno_stage(const LsBatchGrid* grids, const int* blk_start, int ngrid, in
   const LsBatchGrid& G = grids[gi];
   const PDims g{G.nx + 2 * GC, G.ny + 2 * GC, G.nz + 2 * GC};
   const long long NG = g.N();
-  const long long id = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  const long long id = (long long)((int)xcd_block(blockIdx.x, gridDim.x) - blk_start[gi]) * 256 + threadIdx.x;
   if (id >= NG) return;
   const double* src = stage == 0 ? G.P : (stage == 1 ? G.T1 : G.T2);
   double* out = stage == 0 ? G.T1 : (stage == 1 ? G.T2 : G.P);
@@ -340,11 +335,13 @@ __global__ void __launch_bounds__(256)
 k_batch_unpad(const LsBatchGrid* grids, const int* blk_start, int ngrid) {
   const int gi = batch_grid(blk_start, ngrid);
   const LsBatchGrid& G = grids[gi];
-  const long long p = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  const long long p = (long long)((int)xcd_block(blockIdx.x, gridDim.x) - blk_start[gi]) * 256 + threadIdx.x;
   const long long N = (long long)G.nx * G.ny * G.nz;
   if (p >= N) return;
   const int gx = G.nx + 2 * GC, gy = G.ny + 2 * GC;
-  const int i = (int)(p % G.nx), j = (int)((p / G.nx) % G.ny), k = (int)(p / ((long long)G.nx * G.ny));
+  const unsigned up = (unsigned)p, row = up / (unsigned)G.nx;
+  const int i = (int)(up - row * (unsigned)G.nx), j = (int)(row % (unsigned)G.ny),
+            k = (int)(row / (unsigned)G.ny);
   G.phi[p] = G.P[(long long)(i + GC) + (long long)gx * ((long long)(j + GC) + (long long)gy * (k + GC))];
 }
 
@@ -354,9 +351,11 @@ k_batch_pad_metrics(const LsBatchGrid* grids, const int* blk_start, int ngrid,
   const int gi = batch_grid(blk_start, ngrid);
   const LsBatchGrid& G = grids[gi];
   const PDims g{G.nx + 2 * GC, G.ny + 2 * GC, G.nz + 2 * GC};
-  const long long p = (long long)(blockIdx.x - blk_start[gi]) * 256 + threadIdx.x;
+  const long long p = (long long)((int)xcd_block(blockIdx.x, gridDim.x) - blk_start[gi]) * 256 + threadIdx.x;
   if (p >= g.N()) return;
-  const int i = (int)(p % g.x), j = (int)((p / g.x) % g.y), k = (int)(p / ((long long)g.x * g.y));
+  const unsigned up = (unsigned)p, row = up / (unsigned)g.x;
+  const int i = (int)(up - row * (unsigned)g.x), j = (int)(row % (unsigned)g.y),
+            k = (int)(row / (unsigned)g.y);
   const long long N = (long long)G.nx * G.ny * G.nz, NG = g.N();
   const long long src = mirror(i, G.nx) + (long long)G.nx * (mirror(j, G.ny) +
                                                              (long long)G.ny * mirror(k, G.nz));

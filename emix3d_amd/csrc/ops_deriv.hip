@@ -56,7 +56,11 @@ int op_deriv(emx_ctx* ctx, int dir, const double* f, double* df, double ih) {
 static int deriv_fields(emx_ctx* ctx, int dir, int m, const double* const* in,
                         double* const* out, double ih) {
   int f = 0;
-  for (; f + 1 < m; f += 2) {
+  // pairs of fields per launch on the wavefront kind; one field per launch
+  // on the strided kinds, whose single-field form moves more bytes per
+  // second (fewer registers, more workgroups in flight)
+  const bool pairs = ctx->plan[PLAN_DERIV][dir].kind == KIND_WAVE;
+  for (; pairs && f + 1 < m; f += 2) {
     ProbDeriv<2> p;
     p.in[0] = in[f];
     p.in[1] = in[f + 1];
@@ -64,7 +68,7 @@ static int deriv_fields(emx_ctx* ctx, int dir, int m, const double* const* in,
     p.out[1] = out[f + 1];
     EMX_TRY(launch_lines(ctx, dir, p, ih));
   }
-  if (f < m) {
+  for (; f < m; f++) {
     ProbDeriv<1> p;
     p.in[0] = in[f];
     p.out[0] = out[f];

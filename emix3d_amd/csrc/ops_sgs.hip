@@ -54,109 +54,6 @@ __global__ void k_sgs_geometry(const double* x, Dims d, double* sgd,
   if (threadIdx.x == 0) blockmax[blockIdx.x] = red[0];
 }
 
-// Structure function F2 and the pre-selection eddy viscosity plus the
-// vorticity used by SELECTIVO: solver/sgdm_incomp.f90:129-137 (f_structura
-// :180-298 and amu0*sqrt(F2)*dxyzsgd) and VORT :330-341.
-// Threads own (i,j) columns and march in k over a slab of kz planes.
-__global__ void __launch_bounds__(256)
-k_sgs_structure(const double* u, const double* dvel, const double* jbn,
-                const double* sgd, Dims d, int kz, double* amut, double* w) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= d.nx || j >= d.ny) return;
-  const long long N = d.N();
-  const long long sx = 1, sy = d.nx, sz = (long long)d.nx * d.ny;
-  const double amu0 = 0.1781;
-  const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, d.nz);
-  const double* U1 = u;
-  const double* U2 = u + N;
-  const double* U3 = u + 2 * N;
-  for (int k = k0; k < k1; k++) {
-    const long long p = i * sx + j * sy + k * sz;
-    long long a, b;
-    double s, du, dv, dw, fac;
-#define TERM(A, B, FAC)                                           \
-  du = (U1[A] - U1[B]) * (FAC);                                   \
-  dv = (U2[A] - U2[B]) * (FAC);                                   \
-  dw = (U3[A] - U3[B]) * (FAC);
-    // x+
-    if (i < d.nx - 1) { a = p + sx; b = p; } else { a = p; b = p - sx; }
-    fac = sgd[N + p];
-    TERM(a, b, fac);
-    s = du * du + dv * dv + dw * dw;
-    // x-
-    if (i > 0) { a = p - sx; b = p; } else { a = p + sx; b = p; }
-    fac = sgd[2 * N + p];
-    TERM(a, b, fac);
-    s = s + du * du + dv * dv + dw * dw;
-    // y+
-    if (j < d.ny - 1) { a = p + sy; b = p; } else { a = p; b = p - sy; }
-    fac = sgd[3 * N + p];
-    TERM(a, b, fac);
-    s = s + du * du + dv * dv + dw * dw;
-    // y-
-    if (j > 0) { a = p - sy; b = p; } else { a = p; b = p + sy; }
-    fac = sgd[4 * N + p];
-    TERM(a, b, fac);
-    s = s + du * du + dv * dv + dw * dw;
-    // z+
-    if (k < d.nz - 1) { a = p + sz; b = p; } else { a = p; b = p - sz; }
-    fac = sgd[5 * N + p];
-    TERM(a, b, fac);
-    s = s + du * du + dv * dv + dw * dw;
-    // z-
-    if (k > 0) { a = p - sz; b = p; } else { a = p; b = p + sz; }
-    fac = sgd[6 * N + p];
-    TERM(a, b, fac);
-    s = s + du * du + dv * dv + dw * dw;
-#undef TERM
-    const double f2 = s / 6.e0;
-    amut[p] = amu0 * sqrt(f2) * sgd[p];
-    const double j1 = jbn[p], j5 = jbn[4 * N + p], j9 = jbn[8 * N + p];
-    w[p] = j5 * dvel[7 * N + p] - j9 * dvel[5 * N + p];
-    w[N + p] = j9 * dvel[2 * N + p] - j1 * dvel[6 * N + p];
-    w[2 * N + p] = j1 * dvel[3 * N + p] - j5 * dvel[N + p];
-  }
-}
-
-// SELECTIVO (solver/sgdm_incomp.f90:360-392) and viscosidad
-// (solver/viscosidad_incomp.f90:12-20).
-__global__ void __launch_bounds__(256)
-k_sgs_select(const double* w, Dims d, int kz, double c4, double* amut,
-             double* vis, double* vis5) {
-  const int i = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  if (i >= d.nx || j >= d.ny) return;
-  const long long N = d.N();
-  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
-  const double cutoff = cos(20.0 * atan(1.0) / 45.0);
-  const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, d.nz);
-  const double* wx = w;
-  const double* wy = w + N;
-  const double* wz = w + 2 * N;
-  for (int k = k0; k < k1; k++) {
-    const long long p = i + j * sy + k * sz;
-    const long long n1 = (i == d.nx - 1) ? p : p + 1;
-    const long long n2 = (i == 0) ? p : p - 1;
-    const long long n3 = (k == d.nz - 1) ? p : p + sz;
-    const long long n4 = (k == 0) ? p : p - sz;
-    const long long n5 = (j == d.ny - 1) ? p : p + sy;
-    const long long n6 = (j == 0) ? p : p - sy;
-    const double wxn = wx[n1] + wx[n2] + wx[n3] + wx[n4] + wx[n5] + wx[n6];
-    const double wyn = wy[n1] + wy[n2] + wy[n3] + wy[n4] + wy[n5] + wy[n6];
-    const double wzn = wz[n1] + wz[n2] + wz[n3] + wz[n4] + wz[n5] + wz[n6];
-    const double a = wx[p], b = wy[p], c = wz[p];
-    double wn = a * a + b * b + c * c;
-    wn = sqrt(wn * (wxn * wxn + wyn * wyn + wzn * wzn));
-    const double s1 = (a * wxn + b * wyn + c * wzn) / wn;
-    double m = amut[p];
-    if (s1 > cutoff) m = 0.0;
-    amut[p] = m;
-    if (vis5) vis5[p] = m / 0.7;
-    vis[p] = c4 * 1.0 + m;
-  }
-}
-
 int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
                     double* dmins, double* blockmax, int nblocks) {
   hipLaunchKernelGGL(k_sgs_geometry, dim3(nblocks), dim3(256), 0, ctx->stream,
@@ -165,21 +62,139 @@ int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
   return EMX_OK;
 }
 
+// viscosidad_t + viscosidad in one pass: f_structura
+// (solver/sgdm_incomp.f90:180-298), amut = amu0*sqrt(F2)*dxyzsgd (:129-137),
+// VORT (:330-341), SELECTIVO (:360-392) and viscosidad
+// (solver/viscosidad_incomp.f90:12-20), in the reference's operation order.  A workgroup owns a 32 x 8 tile of (i, j) and marches k
+// through a slab; the velocity and the vorticity of planes k-1, k, k+1 are
+// held in LDS with a one-point halo in i and j (rolling over three slots),
+// so every input is read from memory once per workgroup instead of once
+// per neighbour, and the vorticity never goes to memory.
+constexpr int SG_TX = 32, SG_TY = 8, SG_HX = SG_TX + 2, SG_HY = SG_TY + 2,
+              SG_HP = SG_HX * SG_HY;
+
+__global__ void __launch_bounds__(256)
+k_sgs_fused(const double* u, const double* dvel, const double* jbn, const double* sgd,
+            Dims d, int kz, double c4, double* amut, double* vis, double* vis5) {
+  __shared__ double su[3][3][SG_HP];  // [slot][component][halo point]
+  __shared__ double sw[3][3][SG_HP];
+  const long long N = d.N();
+  const long long sy = d.nx, sz = (long long)d.nx * d.ny;
+  const int tx = threadIdx.x % SG_TX, ty = threadIdx.x / SG_TX;
+  const int i0 = blockIdx.x * SG_TX, j0 = blockIdx.y * SG_TY;
+  const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, d.nz);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i < d.nx && j < d.ny;
+  const double amu0 = 0.1781;
+  const double cutoff = cos(20.0 * atan(1.0) / 45.0);
+  // plane kk (clamped into the grid) into slot kk mod 3; halo points
+  // outside the grid hold clamped copies and are never used
+  auto load_plane = [&](int kk) {
+    const int slot = ((kk % 3) + 3) % 3;
+    const int gk = kk < 0 ? 0 : (kk >= d.nz ? d.nz - 1 : kk);
+    for (int h = threadIdx.x; h < SG_HP; h += blockDim.x) {
+      int gi = i0 - 1 + h % SG_HX, gj = j0 - 1 + h / SG_HX;
+      gi = gi < 0 ? 0 : (gi >= d.nx ? d.nx - 1 : gi);
+      gj = gj < 0 ? 0 : (gj >= d.ny ? d.ny - 1 : gj);
+      const long long q = gi + gj * sy + gk * sz;
+#pragma unroll
+      for (int c = 0; c < 3; c++) su[slot][c][h] = u[c * N + q];
+      const double j1 = jbn[q], j5 = jbn[4 * N + q], j9 = jbn[8 * N + q];
+      sw[slot][0][h] = j5 * dvel[7 * N + q] - j9 * dvel[5 * N + q];
+      sw[slot][1][h] = j9 * dvel[2 * N + q] - j1 * dvel[6 * N + q];
+      sw[slot][2][h] = j1 * dvel[3 * N + q] - j5 * dvel[N + q];
+    }
+  };
+  load_plane(k0 - 1);
+  load_plane(k0);
+  const int hc = (ty + 1) * SG_HX + (tx + 1);  // own halo index
+  for (int k = k0; k < k1; k++) {
+    load_plane(k + 1);
+    __syncthreads();
+    if (own) {
+      const long long p = i + j * sy + k * sz;
+      const int sm = ((k - 1) % 3 + 3) % 3, s0 = k % 3, sp = (k + 1) % 3;
+      // structure function, six one-sided differences
+      int a_s, a_h, b_s, b_h;  // (slot, halo index) of the two points
+      double s = 0.0, du, dv, dw, fac;
+#define TERM(FAC)                                                           \
+  du = (su[a_s][0][a_h] - su[b_s][0][b_h]) * (FAC);                         \
+  dv = (su[a_s][1][a_h] - su[b_s][1][b_h]) * (FAC);                         \
+  dw = (su[a_s][2][a_h] - su[b_s][2][b_h]) * (FAC);
+      // x+
+      if (i < d.nx - 1) { a_s = s0; a_h = hc + 1; b_s = s0; b_h = hc; }
+      else { a_s = s0; a_h = hc; b_s = s0; b_h = hc - 1; }
+      fac = sgd[N + p];
+      TERM(fac);
+      s = du * du + dv * dv + dw * dw;
+      // x-
+      if (i > 0) { a_s = s0; a_h = hc - 1; b_s = s0; b_h = hc; }
+      else { a_s = s0; a_h = hc + 1; b_s = s0; b_h = hc; }
+      fac = sgd[2 * N + p];
+      TERM(fac);
+      s = s + du * du + dv * dv + dw * dw;
+      // y+
+      if (j < d.ny - 1) { a_s = s0; a_h = hc + SG_HX; b_s = s0; b_h = hc; }
+      else { a_s = s0; a_h = hc; b_s = s0; b_h = hc - SG_HX; }
+      fac = sgd[3 * N + p];
+      TERM(fac);
+      s = s + du * du + dv * dv + dw * dw;
+      // y-
+      if (j > 0) { a_s = s0; a_h = hc - SG_HX; b_s = s0; b_h = hc; }
+      else { a_s = s0; a_h = hc; b_s = s0; b_h = hc + SG_HX; }
+      fac = sgd[4 * N + p];
+      TERM(fac);
+      s = s + du * du + dv * dv + dw * dw;
+      // z+
+      if (k < d.nz - 1) { a_s = sp; a_h = hc; b_s = s0; b_h = hc; }
+      else { a_s = s0; a_h = hc; b_s = sm; b_h = hc; }
+      fac = sgd[5 * N + p];
+      TERM(fac);
+      s = s + du * du + dv * dv + dw * dw;
+      // z-
+      if (k > 0) { a_s = sm; a_h = hc; b_s = s0; b_h = hc; }
+      else { a_s = s0; a_h = hc; b_s = sp; b_h = hc; }
+      fac = sgd[6 * N + p];
+      TERM(fac);
+      s = s + du * du + dv * dv + dw * dw;
+#undef TERM
+      const double f2 = s / 6.e0;
+      double m = amu0 * sqrt(f2) * sgd[p];
+      // SELECTIVO: neighbours n1..n6 = x+, x-, z+, z-, y+, y- (self at walls)
+      const int n1s = s0, n1h = i == d.nx - 1 ? hc : hc + 1;
+      const int n2s = s0, n2h = i == 0 ? hc : hc - 1;
+      const int n3s = k == d.nz - 1 ? s0 : sp, n3h = hc;
+      const int n4s = k == 0 ? s0 : sm, n4h = hc;
+      const int n5s = s0, n5h = j == d.ny - 1 ? hc : hc + SG_HX;
+      const int n6s = s0, n6h = j == 0 ? hc : hc - SG_HX;
+      double wn[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+        wn[c] = sw[n1s][c][n1h] + sw[n2s][c][n2h] + sw[n3s][c][n3h] + sw[n4s][c][n4h] +
+                sw[n5s][c][n5h] + sw[n6s][c][n6h];
+      const double a = sw[s0][0][hc], b = sw[s0][1][hc], c3 = sw[s0][2][hc];
+      double wnn = a * a + b * b + c3 * c3;
+      wnn = sqrt(wnn * (wn[0] * wn[0] + wn[1] * wn[1] + wn[2] * wn[2]));
+      const double s1 = (a * wn[0] + b * wn[1] + c3 * wn[2]) / wnn;
+      if (s1 > cutoff) m = 0.0;
+      amut[p] = m;
+      if (vis5) vis5[p] = m / 0.7;
+      vis[p] = c4 * 1.0 + m;
+    }
+    __syncthreads();  // the slot of plane k-1 is reloaded next
+  }
+}
+
 int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
             const double* jbn, const double* sgd, double c4, double* amut,
             double* vis, double* vis5) {
   const Dims d = ctx->d;
-  const int kz = 8;
-  dim3 grid((d.nx + 63) / 64, (d.ny + 3) / 4, (d.nz + kz - 1) / kz);
-  dim3 block(64, 4);
-  double* w = ctx->sgs_tmp;
-  EMX_TIMED(ctx, "sgdm_structure", 184.0 * d.N(),
-          hipLaunchKernelGGL(k_sgs_structure, grid, block, 0, ctx->stream, u, dvel,
-                     jbn, sgd, d, kz, amut, w));
-  EMX_TRY_HIP(hipGetLastError());
-  EMX_TIMED(ctx, "sgdm_select", (vis5 ? 56.0 : 48.0) * d.N(),
-          hipLaunchKernelGGL(k_sgs_select, grid, block, 0, ctx->stream, w, d, kz, c4,
-                     amut, vis, vis5));
+  const int kz = 16;
+  dim3 grid((d.nx + SG_TX - 1) / SG_TX, (d.ny + SG_TY - 1) / SG_TY, (d.nz + kz - 1) / kz);
+  // reads u (3), dvel (6), jbn (3), sgd (7); writes amut, vis (, vis5)
+  EMX_TIMED(ctx, "sgdm", (vis5 ? 176.0 : 168.0) * d.N(),
+            hipLaunchKernelGGL(k_sgs_fused, grid, dim3(256), 0, ctx->stream, u, dvel, jbn,
+                               sgd, d, kz, c4, amut, vis, vis5));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }
