@@ -35,7 +35,8 @@ int make_plan(emx_ctx* ctx, int op, int dir, int kind, int C) {
     return EMX_ERR_ARG;
   }
   LineTab t{};
-  EMX_TRY(get_line_tab(ctx->scheme, n, C, kind, &t));
+  if (kind != KIND_MATERIALIZED)  // that form solves with the derivative plan
+    EMX_TRY(get_line_tab(ctx->scheme, n, C, kind, &t));
   ctx->plan[op][dir].tab = t;
   ctx->plan[op][dir].kind = kind;
   ctx->plan[op][dir].C = C;
@@ -46,6 +47,7 @@ void free_ctx(emx_ctx* c) {
   if (!c) return;
   (void)hipFree(c->acc);
   (void)hipFree(c->sgs_tmp);
+  (void)hipFree(c->fluxbuf);
   (void)hipFree(c->mlp_pack);
   (void)hipFree(c->ls_buf);
   (void)hipFree(c->stage);
@@ -119,6 +121,7 @@ int emx_create(emx_ctx** out, int nx, int ny, int nz, int scheme,
   c->ls_bytes = ls_scratch_doubles(c->d) * sizeof(double);
   if (hipMalloc(&c->acc, 8 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->sgs_tmp, 2 * N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->fluxbuf, 8 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->mlp_pack, mlp_pack_elems() * 2) != hipSuccess ||
       hipMalloc(&c->ls_buf, c->ls_bytes) != hipSuccess) {
     (void)hipGetLastError();

@@ -70,6 +70,15 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Orders the LDS traffic of one wavefront: its writes are visible to its
+// own later reads (other lanes included).  The wavefront kind exchanges data
+// only inside a wave, so it needs no workgroup barrier per field.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Line group of workgroup bxp (of gx along the lines) for LG lines per
 // wavefront.  A group of 8 doubles (64 bytes) shares each 128-byte cache
 // line with its neighbour.  Workgroups go to the eight XCDs round robin,
@@ -96,9 +105,10 @@ __device__ __forceinline__ int line_group(int bxp, int gx) {
 // LDS buffer and is solved on its own (few registers for many fields).
 // The Thomas factors are staged in LDS once per workgroup (4 lines).
 template <int S, int C, class Prob>
-__global__ void __launch_bounds__(256)
-k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
-             long long sk, long long sl, double ih) {
+__device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab& T,
+                                                long long nlines, int nyl, long long sj,
+                                                long long sk, long long sl, double ih,
+                                                long long blk) {
   constexpr int NV = Prob::NV;
   constexpr int W = C + 6;
   constexpr int STR = C + 1;  // chunk stride in the transpose buffer
@@ -114,7 +124,7 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
   double* s_ab = s_af + 384;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double* xb = s_ab + 384 + wv * 64 * STR;
-  const long long L = (long long)blockIdx.x * 4 + wv;
+  const long long L = blk * 4 + wv;
   const bool lact = L < nlines;
   const long long Lc = lact ? L : nlines - 1;
   const long long base = (Lc % nyl) * sj + (Lc / nyl) * sk;
@@ -129,19 +139,20 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
   }
   for (int t = threadIdx.x; t < ntab + 768; t += 256)
     lds[t] = T.blob[t < ntab ? t : t - ntab + (int)(T.Af - T.blob)];
+  __syncthreads();  // the tables; past here each wave only touches its own xb
 
   const int s = lane * C;
   const bool act = lane < P;
 #pragma unroll
   for (int a = 0; a < NV; a++) {
     // to the chunk order
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int q = 0; q < C; q++) {
       const int idx = q * 64 + lane;
       xb[idx + idx / C] = v[q][a];
     }
-    __syncthreads();
+    wave_sync();
     double w[W];
 #pragma unroll
     for (int q = 0; q < W; q++) {
@@ -207,25 +218,32 @@ k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
       }
     }
     // back to the coalesced order
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int q = 0; q < C; q++) {
       const int l = s + q;
       if (act && l < n) xb[l + l / C] = y[q];
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int q = 0; q < C; q++) {
       const int idx = q * 64 + lane;
       v[q][a] = xb[idx + idx / C];
     }
   }
-  if (!lact) return;  // after the last barrier
+  if (!lact) return;
 #pragma unroll
   for (int q = 0; q < C; q++) {
     const int idx = q * 64 + lane;
     if (idx < n) prob.store(base + (long long)idx * sl, v[q]);
   }
+}
+
+template <int S, int C, class Prob>
+__global__ void __launch_bounds__(256)
+k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
+             long long sk, long long sl, double ih) {
+  lines_wave_body<S, C>(prob, T, nlines, nyl, sj, sk, sl, ih, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -889,9 +907,8 @@ inline int max_chunks(int kind) {
 // Default plans.  Derivatives: xi lines one wave per line with the smallest
 // C that fits 64 lanes; eta/zeta lines 16 per wave, C=8 (lines up to 512
 // points), else 64 per wave with the smallest C giving <= 16 waves.  Flux
-// divergence: the same xi shape; eta 16 per wave with C=4 (the tile form,
-// lines up to 256 points), zeta 32 per wave with C=8 (up to 256), else 16
-// per wave with C=8 (up to 512).
+// divergence: the same xi shape; eta/zeta the materialized form (projected
+// fluxes into scratch, then single-field solves on the derivative plan).
 inline void default_plan(int op, int dir, int n, int* kind, int* C) {
   if (dir == 0) {
     *kind = KIND_WAVE;
@@ -899,10 +916,9 @@ inline void default_plan(int op, int dir, int n, int* kind, int* C) {
     while (c * 64 < n) c *= 2;
     *C = c;
   } else if (op == PLAN_FLUX) {
-    // zeta lines (the largest stride) measure faster with 32 lines per
-    // wavefront and C = 8 on MI355X
-    *kind = (dir == 2 && n <= 256) ? KIND_STRIDED32 : KIND_STRIDED16;
-    *C = (n <= 256 && dir != 2) ? 4 : 8;
+    // eta/zeta: the materialized form measures fastest on MI355X
+    *kind = KIND_MATERIALIZED;
+    *C = 0;
   } else if (n <= 512) {
     *kind = KIND_STRIDED16;
     *C = 8;
@@ -916,6 +932,7 @@ inline void default_plan(int op, int dir, int n, int* kind, int* C) {
 
 // Shapes each plan may take.
 inline bool valid_plan(int op, int kind, int C) {
+  if (kind == KIND_MATERIALIZED) return op == PLAN_FLUX && C == 0;
   if (!valid_chunk(kind, C)) return false;
   if (op == PLAN_FLUX) {
     if (kind == KIND_WAVE) return C <= 16;

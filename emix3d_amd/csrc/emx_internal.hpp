@@ -44,6 +44,10 @@ enum LineKind {
   KIND_STRIDED32 = 2,  // 32 lines per wave, two chunk slots per wave
   KIND_STRIDED16 = 3,  // 16 lines per wave, four chunk slots per wave
   KIND_STRIDED8 = 4,   // 8 lines per wave, eight chunk slots per wave
+  // flux divergence only: the projected fluxes of the direction formed by
+  // one pointwise kernel into scratch, then one single-field solve per flux
+  // with the direction's derivative plan
+  KIND_MATERIALIZED = 5,
 };
 
 // Device tables of one (scheme, n, chunk) combination.  All arrays live in
@@ -144,6 +148,7 @@ struct emx_ctx {
   // persistent scratch (device): eight accumulators for the fused
   // flux-divergence, two fields for the dynamic eddy-viscosity model.
   double* acc = nullptr;       // 8*N
+  double* fluxbuf = nullptr;   // 8*N projected fluxes (materialized form)
   double* sgs_tmp = nullptr;   // 2*N (dynamic-model contractions, flags)
   void* mlp_pack = nullptr;    // BF16 operands of the closure network
   // level-set scratch (allocated lazily on first level-set call)

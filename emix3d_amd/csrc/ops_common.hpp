@@ -254,6 +254,26 @@ struct ProbDivAll {
   }
 };
 
+// One projected flux (already J*(metric*flux)) through the derivative and
+// the three-pass accumulation of ProbDivAll (same arithmetic): the
+// materialized form of the flux divergence.
+struct ProbAccum {
+  static constexpr int NV = 1;
+  static const char* label() { return "flux_div_all[accumulate]"; }
+  double bytes_pt() const { return 8.0 * (2 + (pass > 0 ? 1 : 0) + (pass == 2 ? 1 : 0)); }
+  const double* f;
+  double* acc;
+  double* out;       // written in the last pass
+  const double* ij;  // 1/J
+  int pass;
+  __device__ __forceinline__ void load(long long p, double* v) const { v[0] = f[p]; }
+  __device__ __forceinline__ void store(long long p, const double* v) const {
+    if (pass == 0) acc[p] = v[0];
+    else if (pass == 1) acc[p] = acc[p] + v[0];
+    else out[p] = ij[p] * (acc[p] + v[0]);
+  }
+};
+
 inline unsigned pointwise_blocks(long long n) {
   long long b = (n + 255) / 256;
   if (b > 65536) b = 65536;

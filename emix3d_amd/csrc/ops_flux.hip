@@ -27,6 +27,23 @@ __global__ void k_flujos(FluxIn F, int neq, double* flux) {
   }
 }
 
+// The eight projected fluxes J*(metric_dir*flux) of direction dir (slots
+// of ProbDivAll: convective 1..5, viscous 2..4), the arithmetic of
+// ProbDivAll::load.
+__global__ void k_flux_project(FluxIn F, int dir, double* out) {
+  const long long N = F.N;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long long)gridDim.x * blockDim.x) {
+    const double det = F.jbn[9 * N + p];
+    const double met = F.jbn[(4 * dir) * N + p];
+#pragma unroll
+    for (int slot = 0; slot < 8; slot++) {
+      const double f = slot < 5 ? conv_flux(F, slot + 1, dir, p) : visc_flux(F, slot - 3, dir, p);
+      out[slot * N + p] = det * (met * f);
+    }
+  }
+}
+
 int op_flux(emx_ctx* ctx, int neq, const FluxPtrs& fp, double* flux) {
   FluxIn F{fp.um, fp.u, fp.pres, fp.vis, fp.jbn, fp.dvel, ctx->d.N()};
   // inputs: um 2..4 [neq 1]; um 2..4, u(neq-1), pres, vis, jbn 1/5/9 and
@@ -85,9 +102,23 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
   const long long N = ctx->d.N();
   const FluxIn in{fp.um, fp.u, fp.pres, fp.vis, fp.jbn, fp.dvel, N};
   for (int dir = 0; dir < 3; dir++) {
+    const DirPlan& pl = ctx->plan[PLAN_FLUX][dir];
+    if (pl.kind == KIND_MATERIALIZED) {
+      // projected fluxes into scratch (16 inputs read, 8 written), then one
+      // single-field solve per flux on the derivative plan
+      EMX_TIMED(ctx, "flux_project", 24.0 * 8.0 * N,
+                hipLaunchKernelGGL(k_flux_project, dim3(pointwise_blocks(N)), dim3(256), 0,
+                                   ctx->stream, in, dir, ctx->fluxbuf));
+      EMX_TRY_HIP(hipGetLastError());
+      for (int slot = 0; slot < 8; slot++) {
+        double* out = slot < 5 ? rs + slot * N : rsv + (slot - 4) * N;
+        ProbAccum p{ctx->fluxbuf + slot * N, ctx->acc + slot * N, out, fp.jbn + 10 * N, dir};
+        EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
+      }
+      continue;
+    }
     // one launch of all eight solves on the wavefront kind with short
     // chunks (8*C formed values per work-item), else two launches of four
-    const DirPlan& pl = ctx->plan[PLAN_FLUX][dir];
     if (pl.kind == KIND_WAVE && pl.C <= 4) {
       ProbDivAll<2> p{in, dir, dir, ctx->acc, rs, rsv};
       EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));

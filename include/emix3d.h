@@ -75,7 +75,9 @@ int emx_set_stream(emx_ctx* ctx, void* stream);
  * chunks over the lanes and waves of a workgroup; C = points per chunk.
  * Valid for op 0: kind 0 C in {1,2,4,8,16}; kind 1 {8,16,32}; kind 2
  * {4,8,16}; kind 3 {4,8}; kind 4 {4,8}.  Op 1: kind 0 C <= 16, kinds 2 and
- * 3 C in {4,8}, kind 4 C in {2,4}.  Used by the autotuner. */
+ * 3 C in {4,8}, kind 4 C in {2,4}, and kind 5 with C = 0: the fluxes of
+ * the direction projected into scratch by one pointwise kernel, then one
+ * single-field solve each on the derivative plan.  Used by the autotuner. */
 int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C);
 int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C);
 
