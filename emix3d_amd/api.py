@@ -356,6 +356,16 @@ class Operator:
                   self._check(out["vis_bar"]), self._check(out["dvel_bar"], 9, "dvel_bar"))
         return out
 
+    def rk_update(self, sub, um0, um, rs, rsv, dt, um1=None):
+        """SSP-RK3 update of sub-step 1..3 (solver/rungek_ico.f90:87-91,
+        152-153, 216-219); um may be None for sub-step 1."""
+        um1 = self._empty(5) if um1 is None else um1
+        self._stream()
+        _lib.call("emx_rk_update", self._ctx, int(sub), self._check(um1, 5, "um1"),
+                  self._check(um0, 5, "um0"), None if um is None else self._check(um, 5, "um"),
+                  self._check(rs, 5, "rs"), self._check(rsv, 5, "rsv"), float(dt))
+        return um1
+
     def advect_level_set(self, pres, jbn, sobject, dxyz, deltamin):
         """advectLevelSet: extrapolate pres into the solid (in place)."""
         self._stream()

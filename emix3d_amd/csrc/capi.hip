@@ -400,6 +400,24 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
   return op_flux_div_all(ctx, FluxPtrs{um, u, pres, vis, jbn, dvel}, ih, rs, rsv);
 }
 
+int emx_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const double* um,
+                  const double* rs, const double* rsv, double dt) {
+  EMX_TRY(check_ctx(ctx));
+  if (sub < 1 || sub > 3) {
+    set_error("emx_rk_update: sub-step %d outside 1..3", sub);
+    return EMX_ERR_ARG;
+  }
+  CHECK_PTR(um1);
+  CHECK_PTR(um0);
+  CHECK_PTR(rs);
+  CHECK_PTR(rsv);
+  if (sub > 1 && !um) {
+    set_error("emx_rk_update: sub-step %d needs um", sub);
+    return EMX_ERR_ARG;
+  }
+  return op_rk_update(ctx, sub, um1, um0, um, rs, rsv, dt);
+}
+
 int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
                 const double* delta, double* nut, double c4, double* vis) {
   EMX_TRY(check_ctx(ctx));

@@ -28,6 +28,8 @@ int op_flux_div(emx_ctx* ctx, int neq, const FluxPtrs& fp, const double ih[3],
 int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
                     double* rs, double* rsv);
 int op_metrics(emx_ctx* ctx, const double* x, double* jbn, int* bad_dev);
+int op_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const double* um,
+                 const double* rs, const double* rsv, double dt);
 int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
                     double* dmins, double* blockmax, int nblocks);
 int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,

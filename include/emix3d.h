@@ -274,6 +274,16 @@ int emx_flux_div_all_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_b
                          double ihy, double ihz, double* um_bar, double* u_bar,
                          double* pres_bar, double* vis_bar, double* dvel_bar);
 
+/* SSP-RK3 update of the five conservative variables after the flux
+ * divergence of sub-step sub (1..3), solver/rungek_ico.f90:87-91, 152-153,
+ * 216-219: um1 = um0 + dt*(rs+rsv); 0.75*um0 + 0.25*um + 0.25*dt*(rs+rsv);
+ * um0/3 + 2*um/3 + (2/3)*dt*(rs+rsv).  All arrays 5 comps; um may be NULL
+ * for sub-step 1.  (The boundary values, impeller forcing and the
+ * primitive-variable update that follow in RUNGEK are not part of this
+ * library.) */
+int emx_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const double* um,
+                  const double* rs, const double* rsv, double dt);
+
 /* Pressure extrapolation into the solid (sobject > 0): advect_CUDA
  * (src/LevelSet/lsTransportEc.cu:24-197 -> interface.cu:341-436).
  * dx,dy,dz are the computational spacings the reference passes

@@ -1262,3 +1262,19 @@ int orc_mlp_features(int nx, int ny, int nz, const double *dvel, const double *j
     }
     return 0;
 }
+
+
+/* solver/rungek_ico.f90:87-91 (sub-step 1), 152-153 (2), 216-219 (3), in
+ * the Fortran evaluation order. */
+int orc_rk_update(int sub, long n, double *um1, const double *um0, const double *um,
+                  const double *rs, const double *rsv, double dt)
+{
+    if (sub < 1 || sub > 3) return -1;
+    for (long p = 0; p < n; p++) {
+        double r = rs[p] + rsv[p];
+        if (sub == 1) um1[p] = um0[p] + dt * r;
+        else if (sub == 2) um1[p] = 0.75 * um0[p] + 0.25 * um[p] + 0.25 * dt * r;
+        else um1[p] = (1.0 / 3.0) * um0[p] + (2.0 / 3.0) * um[p] + (2.0 / 3.0) * dt * r;
+    }
+    return 0;
+}

@@ -127,6 +127,11 @@ void orc_pad_mirror(int nx, int ny, int nz, int g, const double *in,
 int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gphys,
                 const double *delta, double *nut);
 
+/* SSP-RK3 update of sub-step sub (1..3), solver/rungek_ico.f90:87-91,
+ * 152-153, 216-219, on n values (um may be NULL for sub-step 1). */
+int orc_rk_update(int sub, long n, double *um1, const double *um0, const double *um,
+                  const double *rs, const double *rsv, double dt);
+
 /* ---- learned closure network (no reference counterpart) ------------------ */
 /* 10 -> 64 -> 64 -> 64 -> 1, SiLU hidden activations, softplus output.
  * params (9089): W1[64][10] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]

@@ -61,6 +61,7 @@ def lib():
             "orc_pad_mirror": [_I, _I, _I, _I, _P, _P],
             "orc_weno5_derivs": [_I, _I, _I, _P, _P, _D, _D, _D, _P, _P],
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
+            "orc_rk_update": [_I, _L, _P, _P, _P, _P, _P, _D],
             "orc_mlp_forward": [_P, _P, _P, _L, _P],
             "orc_mlp_grad": [_P, _P, _P, _P, _L, _P],
             "orc_mlp_features": [_I, _I, _I, _P, _P, _P, _P],
@@ -167,6 +168,18 @@ def flux_div_all_adj(scheme, rs_bar, rsv_bar, um, u, vis, jbn, dvel, ih):
                                       _ptr(bars["vis_bar"]), _ptr(bars["dvel_bar"])),
            "flux_div_all_adj")
     return bars
+
+
+def rk_update(sub, um0, um, rs, rsv, dt):
+    """SSP-RK3 update of sub-step sub (um may be None for sub-step 1)."""
+    um0 = np.ascontiguousarray(um0, dtype=np.float64)
+    out = np.empty_like(um0)
+    umc = None if um is None else np.ascontiguousarray(um, dtype=np.float64)
+    _check(lib().orc_rk_update(sub, um0.size, _ptr(out), _ptr(um0),
+                               None if umc is None else _ptr(umc),
+                               _ptr(np.ascontiguousarray(rs, dtype=np.float64)),
+                               _ptr(np.ascontiguousarray(rsv, dtype=np.float64)), dt), "rk_update")
+    return out
 
 
 def _f32(a):

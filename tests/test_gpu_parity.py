@@ -354,3 +354,32 @@ def test_full_size_lines_against_oracle(oracle):
             ref = oracle.deriv_line(2, line, float(n - 1))
             assert normwise_rel(gl, ref) <= TOL
     print("  256^3 sampled lines ok", flush=True)
+
+
+def test_rk_update_sub_steps(oracle):
+    """The three SSP-RK3 sub-step updates against the oracle (chained, each
+    sub-step fed the previous GPU result), in place (um1 aliasing um), and
+    the argument checks."""
+    from emix3d_amd._lib import EmxError
+    shape = (37, 19, 23)
+    nx, ny, nz = shape
+    rng = np.random.default_rng(91)
+    um0 = rng.standard_normal((5, nz, ny, nx))
+    rs = [rng.standard_normal((5, nz, ny, nx)) for _ in range(3)]
+    rsv = [rng.standard_normal((5, nz, ny, nx)) for _ in range(3)]
+    op = Operator(*shape, "c6c")
+    dt = 3.7e-3
+    um = None
+    for sub in (1, 2, 3):
+        got = host(op.rk_update(sub, dev(um0), None if um is None else dev(um), dev(rs[sub - 1]),
+                                dev(rsv[sub - 1]), dt))
+        ref = oracle.rk_update(sub, um0, um, rs[sub - 1], rsv[sub - 1], dt)
+        report(f"rk sub-step {sub}", normwise_rel(got, ref))
+        um = got
+    inplace = dev(um)
+    op.rk_update(3, dev(um0), inplace, dev(rs[2]), dev(rsv[2]), dt, um1=inplace)
+    report("rk in place", normwise_rel(host(inplace), oracle.rk_update(3, um0, um, rs[2], rsv[2],
+                                                                       dt)))
+    for sub, u in ((0, None), (4, dev(um)), (2, None)):
+        with pytest.raises(EmxError):
+            op.rk_update(sub, dev(um0), u, dev(rs[0]), dev(rsv[0]), dt)

@@ -266,3 +266,23 @@ def test_extrapolation_is_upwind_transport_along_the_normal(oracle):
     mid = slice(35, n - 35)
     ex = p[:, :, mid] - 10 * (0.5 * dmin) * b
     assert normwise_rel(out[:, :, mid], ex) < 1e-13
+
+
+def test_rk_update_is_ssp_rk3(oracle):
+    """The three RUNGEK sub-step updates (solver/rungek_ico.f90:87-91,
+    152-153, 216-219) applied to du/dt = lam*u give u0 * (1 + z + z^2/2 +
+    z^3/6), z = lam*dt: the third-order Shu-Osher scheme.  A wrong
+    coefficient in any sub-step breaks the cubic."""
+    for z in (0.3, -0.7, 1.1):
+        lam, dt = z / 0.05, 0.05
+        u0 = np.array([1.0, 2.0, -0.5])
+        zero = np.zeros_like(u0)
+        u1 = oracle.rk_update(1, u0, None, lam * u0, zero, dt)
+        u2 = oracle.rk_update(2, u0, u1, lam * u1, zero, dt)
+        u3 = oracle.rk_update(3, u0, u2, lam * u2, zero, dt)
+        ref = u0 * (1 + z + z * z / 2 + z ** 3 / 6)
+        assert np.allclose(u3, ref, rtol=1e-14, atol=0)
+    # rs and rsv enter as their sum
+    a = oracle.rk_update(2, u0, u0 * 3, u0, u0 * 2, 0.1)
+    b = oracle.rk_update(2, u0, u0 * 3, u0 * 3, zero, 0.1)
+    assert np.allclose(a, b, rtol=1e-15)
