@@ -26,6 +26,7 @@ SIGNATURES = {
     "emx_set_plan": ([_P, _I, _I, _I, _I], _I),
     "emx_get_plan": ([_P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "emx_timing_begin": ([_P], _I),
+    "emx_timing_select": ([_P, ctypes.c_char_p], _I),
     "emx_timing_end": ([_P, ctypes.POINTER(_I)], _I),
     "emx_timing_entry": ([_P, _I, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_I),
                           ctypes.POINTER(_I), ctypes.POINTER(_D), ctypes.POINTER(_D)], _I),

@@ -140,9 +140,11 @@ class Operator:
         return k.value, c.value
 
     # ---------------------------------------------------------------- timing
-    def timing_begin(self):
-        """Start recording every launch of this context with HIP events on
-        its stream (emx_timing_begin)."""
+    def timing_begin(self, label=None):
+        """Start recording every launch of this context (or, with label,
+        only the launches of that kernel) with HIP events on its stream
+        (emx_timing_select, emx_timing_begin)."""
+        _lib.call("emx_timing_select", self._ctx, None if label is None else label.encode())
         _lib.call("emx_timing_begin", self._ctx)
 
     def timing_end(self):

@@ -43,6 +43,27 @@ int make_plan(emx_ctx* ctx, int op, int dir, int kind, int C) {
   return EMX_OK;
 }
 
+// Grow the projected-flux scratch to 8 fields per direction on the
+// materialized kind of the flux-divergence plan.
+int ensure_fluxbuf(emx_ctx* c) {
+  int need = 0;
+  for (int dir = 0; dir < 3; dir++)
+    if (c->plan[PLAN_FLUX][dir].kind == KIND_MATERIALIZED) need++;
+  if (need <= c->fluxbuf_dirs) return EMX_OK;
+  const size_t N = (size_t)c->d.N();
+  if (c->fluxbuf) EMX_TRY_HIP(hipFree(c->fluxbuf));
+  c->fluxbuf = nullptr;
+  c->fluxbuf_dirs = 0;
+  if (hipMalloc(&c->fluxbuf, (size_t)need * 8 * N * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    c->fluxbuf = nullptr;
+    set_error("device allocation of %d x 8 projected-flux fields of %zu points failed", need, N);
+    return EMX_ERR_NOMEM;
+  }
+  c->fluxbuf_dirs = need;
+  return EMX_OK;
+}
+
 void free_ctx(emx_ctx* c) {
   if (!c) return;
   (void)hipFree(c->acc);
@@ -121,13 +142,16 @@ int emx_create(emx_ctx** out, int nx, int ny, int nz, int scheme,
   c->ls_bytes = ls_scratch_doubles(c->d) * sizeof(double);
   if (hipMalloc(&c->acc, 8 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->sgs_tmp, 2 * N * sizeof(double)) != hipSuccess ||
-      hipMalloc(&c->fluxbuf, 8 * N * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->mlp_pack, mlp_pack_elems() * 2) != hipSuccess ||
       hipMalloc(&c->ls_buf, c->ls_bytes) != hipSuccess) {
     (void)hipGetLastError();
     set_error("emx_create: device allocation of scratch for %zu points failed", N);
     free_ctx(c);
     return EMX_ERR_NOMEM;
+  }
+  if (int rc = ensure_fluxbuf(c); rc != EMX_OK) {
+    free_ctx(c);
+    return rc;
   }
   *out = c;
   return EMX_OK;
@@ -151,7 +175,15 @@ int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C) {
     set_error("emx_set_plan: bad operation class %d or direction %d", op, dir);
     return EMX_ERR_ARG;
   }
-  return make_plan(ctx, op, dir, kind, C);
+  const int old_kind = ctx->plan[op][dir].kind, old_C = ctx->plan[op][dir].C;
+  EMX_TRY(make_plan(ctx, op, dir, kind, C));
+  if (op == PLAN_FLUX) {
+    if (int rc = ensure_fluxbuf(ctx); rc != EMX_OK) {
+      (void)make_plan(ctx, op, dir, old_kind, old_C);
+      return rc;
+    }
+  }
+  return EMX_OK;
 }
 
 int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C) {
@@ -170,6 +202,12 @@ int emx_timing_begin(emx_ctx* ctx) {
   ctx->timed.clear();
   ctx->timing_summary.clear();
   ctx->timing = true;
+  return EMX_OK;
+}
+
+int emx_timing_select(emx_ctx* ctx, const char* label) {
+  EMX_TRY(check_ctx(ctx));
+  ctx->timing_label = label ? label : "";
   return EMX_OK;
 }
 

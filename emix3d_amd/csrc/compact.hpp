@@ -66,6 +66,22 @@ __host__ __device__ __forceinline__ double rhs_row(const double* w, int l, int n
   }
 }
 
+// Problem traits: TILE (many inputs per point, formed once), PREFETCH
+// (load() is plain loads of few fields, so the wavefront kind may keep the
+// next line's loads in flight while it solves the current one).
+template <class Prob, class = void>
+struct uses_tile { static constexpr bool value = false; };
+template <class Prob>
+struct uses_tile<Prob, decltype(void(Prob::TILE))> {
+  static constexpr bool value = Prob::TILE;
+};
+template <class Prob, class = void>
+struct prefetches { static constexpr bool value = false; };
+template <class Prob>
+struct prefetches<Prob, decltype(void(Prob::PREFETCH))> {
+  static constexpr bool value = Prob::PREFETCH;
+};
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
@@ -103,139 +119,177 @@ __device__ __forceinline__ int line_group(int bxp, int gx) {
 // coalesced (lane t touches points t + 64q); the NV values of a point are
 // formed once, then each field moves to the chunk order through a per-wave
 // LDS buffer and is solved on its own (few registers for many fields).
-// The Thomas factors are staged in LDS once per workgroup (4 lines).
+// The Thomas factors are staged in LDS once per workgroup (4 lines), in the
+// chunk order of the lanes: the factor of point l = lane*C + q sits at
+// q*64 + lane, so the 64 lanes read 64 consecutive doubles (no bank
+// conflicts), and the backward factor is stored shifted (entry l holds
+// gamma[l+1]).
 template <int S, int C, class Prob>
 __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab& T,
                                                 long long nlines, int nyl, long long sj,
                                                 long long sk, long long sl, double ih,
-                                                long long blk) {
+                                                long long blk, long long nblk) {
   constexpr int NV = Prob::NV;
+  constexpr bool PRE = prefetches<Prob>::value;
   constexpr int W = C + 6;
   constexpr int STR = C + 1;  // chunk stride in the transpose buffer
+  constexpr int TS = 64 * C;  // per-point table stride (chunk order)
   extern __shared__ double lds[];
   const int n = T.n, P = T.P;
-  const int ntab = 5 * n + 1;
   double* s_ib = lds;
-  double* s_m = s_ib + n;
-  double* s_gam = s_m + n;
-  double* s_H = s_gam + n + 1;
-  double* s_G = s_H + n;
-  double* s_af = s_G + n;    // Af[384] Ab[384]
+  double* s_m = s_ib + TS;
+  double* s_gam1 = s_m + TS;
+  double* s_H = s_gam1 + TS;
+  double* s_G = s_H + TS;
+  double* s_af = s_G + TS;    // Af[384] Ab[384]
   double* s_ab = s_af + 384;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double* xb = s_ab + 384 + wv * 64 * STR;
-  const long long L = blk * 4 + wv;
-  const bool lact = L < nlines;
-  const long long Lc = lact ? L : nlines - 1;
-  const long long base = (Lc % nyl) * sj + (Lc / nyl) * sk;
-
-  // coalesced loads (all issued before anything waits on them)
-  double v[C][NV];
+  const long long stride = (long long)nblk * 4;
+  long long L = blk * 4 + wv;
+  auto line_base = [&](long long Lx) { return (Lx % nyl) * sj + (Lx / nyl) * sk; };
+  // coalesced loads of one line (all issued before anything waits on them)
+  auto load_line = [&](long long Lx, double (&vv)[C][NV]) {
+    const long long bx = line_base(Lx);
 #pragma unroll
-  for (int q = 0; q < C; q++) {
-    const int idx = q * 64 + lane;
-    const int idc = idx < n ? idx : n - 1;
-    prob.load(base + (long long)idc * sl, v[q]);
+    for (int q = 0; q < C; q++) {
+      const int idx = q * 64 + lane;
+      const int idc = idx < n ? idx : n - 1;
+      prob.load(bx + (long long)idc * sl, vv[q]);
+    }
+  };
+  double v[C][NV];
+  if (L < nlines) load_line(L, v);
+  // blob: ib[n] m[n] gamma[n+1] H[n] G[n]; table tb of point l at
+  // tb*n + (tb >= 2) + l (gamma shifted by one)
+  for (int t = threadIdx.x; t < 5 * TS + 768; t += 256) {
+    double val;
+    if (t < 5 * TS) {
+      const int tb = t / TS, r = t % TS, l = (r % 64) * C + r / 64;
+      val = l < n ? T.blob[tb * n + (tb >= 2 ? 1 : 0) + l] : 0.0;
+    } else {
+      val = T.Af[t - 5 * TS];
+    }
+    lds[t] = val;
   }
-  for (int t = threadIdx.x; t < ntab + 768; t += 256)
-    lds[t] = T.blob[t < ntab ? t : t - ntab + (int)(T.Af - T.blob)];
   __syncthreads();  // the tables; past here each wave only touches its own xb
+  if (L >= nlines) return;
 
   const int s = lane * C;
   const bool act = lane < P;
-#pragma unroll
-  for (int a = 0; a < NV; a++) {
-    // to the chunk order
-    wave_sync();
-#pragma unroll
-    for (int q = 0; q < C; q++) {
-      const int idx = q * 64 + lane;
-      xb[idx + idx / C] = v[q][a];
+  // The wave's lines L, L + stride, ...; with PRE the next line's loads are
+  // in flight while the current one is solved.
+  for (;;) {
+    const long long Ln = L + stride;
+    const bool more = Ln < nlines;
+    double vn[C][NV];
+    if constexpr (PRE) {
+      if (more) load_line(Ln, vn);
     }
-    wave_sync();
-    double w[W];
 #pragma unroll
-    for (int q = 0; q < W; q++) {
-      const int l = clampi(s - 3 + q, 0, n - 1);
-      w[q] = xb[l + l / C];
-    }
-    // local forward sweep from a zero carry
-    double y[C];
-    double yp = 0.0;
+    for (int a = 0; a < NV; a++) {
+      // to the chunk order
+      wave_sync();
 #pragma unroll
-    for (int q = 0; q < C; q++) {
-      const int l = s + q;
-      if (act && l < n) {
-        const double r = rhs_row<S>(&w[q + 3], l, n, ih);
-        yp = r * s_ib[l] - s_m[l] * yp;
-        y[q] = yp;
-      } else {
-        y[q] = 0.0;
+      for (int q = 0; q < C; q++) {
+        const int idx = q * 64 + lane;
+        xb[idx + idx / C] = v[q][a];
       }
-    }
-    // forward carries: inclusive weighted scan over lanes
-    {
-      double B = act ? yp : 0.0;
+      wave_sync();
+      double w[W];
 #pragma unroll
-      for (int k = 0; k < 6; k++) {
-        const int d = 1 << k;
-        const double up = __shfl_up(B, d, 64);
-        if (lane >= d) B += s_af[k * 64 + lane] * up;
+      for (int q = 0; q < W; q++) {
+        const int l = clampi(s - 3 + q, 0, n - 1);
+        w[q] = xb[l + l / C];
       }
-      double Y = __shfl_up(B, 1, 64);
-      if (lane == 0) Y = 0.0;
+      // local forward sweep from a zero carry
+      double y[C];
+      double yp = 0.0;
 #pragma unroll
       for (int q = 0; q < C; q++) {
         const int l = s + q;
-        if (act && l < n) y[q] += s_H[l] * Y;
+        if (act && l < n) {
+          const double r = rhs_row<S>(&w[q + 3], l, n, ih);
+          yp = r * s_ib[q * 64 + lane] - s_m[q * 64 + lane] * yp;
+          y[q] = yp;
+        } else {
+          y[q] = 0.0;
+        }
       }
-    }
-    // local backward sweep from a zero carry
-    double xs = 0.0;
+      // forward carries: inclusive weighted scan over lanes
+      {
+        double B = act ? yp : 0.0;
 #pragma unroll
-    for (int q = C - 1; q >= 0; q--) {
-      const int l = s + q;
-      if (act && l < n) {
-        xs = y[q] - s_gam[l + 1] * xs;
-        y[q] = xs;
-      }
-    }
-    // backward carries: weighted scan from the right, then the result
-    {
-      double B = act ? xs : 0.0;
+        for (int k = 0; k < 6; k++) {
+          const int d = 1 << k;
+          const double up = __shfl_up(B, d, 64);
+          if (lane >= d) B += s_af[k * 64 + lane] * up;
+        }
+        double Y = __shfl_up(B, 1, 64);
+        if (lane == 0) Y = 0.0;
 #pragma unroll
-      for (int k = 0; k < 6; k++) {
-        const int d = 1 << k;
-        const double dn = __shfl_down(B, d, 64);
-        if (lane + d < 64) B += s_ab[k * 64 + lane] * dn;
+        for (int q = 0; q < C; q++) {
+          const int l = s + q;
+          if (act && l < n) y[q] += s_H[q * 64 + lane] * Y;
+        }
       }
-      double X = __shfl_down(B, 1, 64);
-      if (lane == 63) X = 0.0;
+      // local backward sweep from a zero carry
+      double xs = 0.0;
+#pragma unroll
+      for (int q = C - 1; q >= 0; q--) {
+        const int l = s + q;
+        if (act && l < n) {
+          xs = y[q] - s_gam1[q * 64 + lane] * xs;
+          y[q] = xs;
+        }
+      }
+      // backward carries: weighted scan from the right, then the result
+      {
+        double B = act ? xs : 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+          const int d = 1 << k;
+          const double dn = __shfl_down(B, d, 64);
+          if (lane + d < 64) B += s_ab[k * 64 + lane] * dn;
+        }
+        double X = __shfl_down(B, 1, 64);
+        if (lane == 63) X = 0.0;
+#pragma unroll
+        for (int q = 0; q < C; q++) {
+          const int l = s + q;
+          if (act && l < n) y[q] += s_G[q * 64 + lane] * X;
+        }
+      }
+      // back to the coalesced order
+      wave_sync();
 #pragma unroll
       for (int q = 0; q < C; q++) {
         const int l = s + q;
-        if (act && l < n) y[q] += s_G[l] * X;
+        if (act && l < n) xb[l + l / C] = y[q];
+      }
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int idx = q * 64 + lane;
+        v[q][a] = xb[idx + idx / C];
       }
     }
-    // back to the coalesced order
-    wave_sync();
-#pragma unroll
-    for (int q = 0; q < C; q++) {
-      const int l = s + q;
-      if (act && l < n) xb[l + l / C] = y[q];
-    }
-    wave_sync();
+    const long long bs = line_base(L);
 #pragma unroll
     for (int q = 0; q < C; q++) {
       const int idx = q * 64 + lane;
-      v[q][a] = xb[idx + idx / C];
+      if (idx < n) prob.store(bs + (long long)idx * sl, v[q]);
     }
-  }
-  if (!lact) return;
+    if (!more) break;
+    L = Ln;
+    if constexpr (PRE) {
 #pragma unroll
-  for (int q = 0; q < C; q++) {
-    const int idx = q * 64 + lane;
-    if (idx < n) prob.store(base + (long long)idx * sl, v[q]);
+      for (int q = 0; q < C; q++)
+#pragma unroll
+        for (int a = 0; a < NV; a++) v[q][a] = vn[q][a];
+    } else {
+      load_line(L, v);
+    }
   }
 }
 
@@ -243,7 +297,7 @@ template <int S, int C, class Prob>
 __global__ void __launch_bounds__(256)
 k_lines_wave(Prob prob, LineTab T, long long nlines, int nyl, long long sj,
              long long sk, long long sl, double ih) {
-  lines_wave_body<S, C>(prob, T, nlines, nyl, sj, sk, sl, ih, blockIdx.x);
+  lines_wave_body<S, C>(prob, T, nlines, nyl, sj, sk, sl, ih, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -691,26 +745,36 @@ inline DirGeom dir_geom(const Dims& d, int dir) {
   return g;
 }
 
+// Workgroups of kernel kern (256 work-items, lds bytes) the device holds at
+// once, over all its compute units.
+template <class K>
+int resident_blocks(K kern, size_t lds) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, lds) != hipSuccess) {
+    (void)hipGetLastError();
+    return 1 << 30;  // unknown: one workgroup per four lines
+  }
+  return (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+}
+
 template <int S, int C, class Prob>
 hipError_t launch_wave_c(const Prob& prob, const LineTab& T, const DirGeom& g,
                          double ih, hipStream_t st) {
-  const long long blocks = (g.nlines + 3) / 4;
-  const size_t lds = (size_t)(5 * T.n + 1 + 768 + 4 * 64 * (C + 1)) * sizeof(double);
+  // one resident set of workgroups; each wave takes every (4*blocks)-th line
+  const size_t lds = (size_t)(5 * 64 * C + 768 + 4 * 64 * (C + 1)) * sizeof(double);
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_lines_wave<S, C, Prob>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  static const int resident = resident_blocks(k_lines_wave<S, C, Prob>, lds);
+  long long blocks = (g.nlines + 3) / 4;
+  if (blocks > resident) blocks = resident;
   hipLaunchKernelGGL((k_lines_wave<S, C, Prob>), dim3((unsigned)blocks),
                      dim3(256), lds, st, prob, T, g.nlines, g.nyl, g.sj, g.sk,
                      g.sl, ih);
   return hipGetLastError();
 }
-
-template <class Prob, class = void>
-struct uses_tile { static constexpr bool value = false; };
-template <class Prob>
-struct uses_tile<Prob, decltype(void(Prob::TILE))> {
-  static constexpr bool value = Prob::TILE;
-};
 
 template <int S, int C, int LG, class Prob>
 hipError_t launch_strided_c(const Prob& prob, const LineTab& T,
@@ -910,15 +974,21 @@ inline int max_chunks(int kind) {
 // divergence: the same xi shape; eta/zeta the materialized form (projected
 // fluxes into scratch, then single-field solves on the derivative plan).
 inline void default_plan(int op, int dir, int n, int* kind, int* C) {
-  if (dir == 0) {
+  if (op == PLAN_FLUX) {
+    // the materialized form (one projection pass for all three directions,
+    // then single-field solves) measures fastest on MI355X
+    *kind = KIND_MATERIALIZED;
+    *C = 0;
+  } else if (dir == 0) {
     *kind = KIND_WAVE;
     int c = 1;
     while (c * 64 < n) c *= 2;
     *C = c;
-  } else if (op == PLAN_FLUX) {
-    // eta/zeta: the materialized form measures fastest on MI355X
-    *kind = KIND_MATERIALIZED;
-    *C = 0;
+  } else if (n >= 128 && n <= 512) {
+    // 32 lines per wavefront, 16-point chunks: the fastest single-field
+    // solve at 256 points per line with a working set beyond the L3
+    *kind = KIND_STRIDED32;
+    *C = 16;
   } else if (n <= 512) {
     *kind = KIND_STRIDED16;
     *C = 8;

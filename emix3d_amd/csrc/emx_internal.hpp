@@ -148,7 +148,8 @@ struct emx_ctx {
   // persistent scratch (device): eight accumulators for the fused
   // flux-divergence, two fields for the dynamic eddy-viscosity model.
   double* acc = nullptr;       // 8*N
-  double* fluxbuf = nullptr;   // 8*N projected fluxes (materialized form)
+  double* fluxbuf = nullptr;   // 8*N per materialized direction: projected fluxes
+  int fluxbuf_dirs = 0;        // directions fluxbuf holds
   double* sgs_tmp = nullptr;   // 2*N (dynamic-model contractions, flags)
   void* mlp_pack = nullptr;    // BF16 operands of the closure network
   // level-set scratch (allocated lazily on first level-set call)
@@ -159,6 +160,7 @@ struct emx_ctx {
   size_t stage_bytes = 0;
   // per-launch timing (off unless emx_timing_begin)
   bool timing = false;
+  std::string timing_label;                 // record only this label (empty: all)
   std::vector<hipEvent_t> ev_pool;          // 2 per recorded launch
   std::vector<emx::TimedLaunch> timed;
   std::vector<emx::TimingEntry> timing_summary;

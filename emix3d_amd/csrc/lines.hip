@@ -265,7 +265,8 @@ int get_line_tab(int scheme, int n, int C, int kind, LineTab* out) {
 // one stops the recording (timing is a diagnostic, never an error of the
 // operator).
 LaunchTimer::LaunchTimer(emx_ctx* c, const char* label, int dir, double bytes)
-    : c_(c), on_(c && c->timing) {
+    : c_(c), on_(c && c->timing &&
+                 (c->timing_label.empty() || c->timing_label == label)) {
   if (!on_) return;
   const size_t need = 2 * (c->timed.size() + 1);
   while (c->ev_pool.size() < need) {

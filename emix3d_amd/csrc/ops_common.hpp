@@ -19,6 +19,7 @@ namespace emx {
 template <int N_>
 struct ProbDeriv {
   static constexpr int NV = N_;
+  static constexpr bool PREFETCH = true;
   static const char* label() { return "deriv"; }
   double bytes_pt() const { return 16.0 * N_; }
   const double* in[N_];
@@ -82,10 +83,39 @@ struct FluxIn {
   __device__ __forceinline__ double D(int c, long long p) const { return dvel[(c - 1) * N + p]; }
 };
 
+// The inputs of every flux at one point, held in registers (accessors as
+// FluxIn, the point argument ignored).  Loads that no flux of the caller
+// uses are dropped by the compiler.
+struct FluxPoint {
+  double um[5], u[3], pres, vis, j[10], d[9];
+  __device__ __forceinline__ FluxPoint(const FluxIn& F, long long p) {
+#pragma unroll
+    for (int m = 1; m < 5; m++) um[m] = F.UM(m + 1, p);
+    um[0] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; m++) u[m] = F.U(m + 1, p);
+    pres = F.pres[p];
+    vis = F.vis[p];
+#pragma unroll
+    for (int c = 0; c < 10; c++) j[c] = F.J(c + 1, p);
+#pragma unroll
+    for (int c = 0; c < 9; c++) d[c] = F.D(c + 1, p);
+  }
+  __device__ __forceinline__ double UM(int m, long long) const { return um[m - 1]; }
+  __device__ __forceinline__ double U(int m, long long) const { return u[m - 1]; }
+  __device__ __forceinline__ double J(int c, long long) const { return j[c - 1]; }
+  __device__ __forceinline__ double D(int c, long long) const { return d[c - 1]; }
+};
+
+__device__ __forceinline__ double pres_of(const FluxIn& F, long long p) { return F.pres[p]; }
+__device__ __forceinline__ double pres_of(const FluxPoint& F, long long) { return F.pres; }
+__device__ __forceinline__ double vis_of(const FluxIn& F, long long p) { return F.vis[p]; }
+__device__ __forceinline__ double vis_of(const FluxPoint& F, long long) { return F.vis; }
+
 // Convective flux of equation neq along dir (0:e, 1:f, 2:g),
 // solver/flujos_compact.f90:21-185.
-__device__ __forceinline__ double conv_flux(const FluxIn& F, int neq, int dir,
-                                            long long p) {
+template <class F_>
+__device__ __forceinline__ double conv_flux(const F_& F, int neq, int dir, long long p) {
   switch (neq) {
     case 1:
       return -F.UM(2 + dir, p);
@@ -93,7 +123,7 @@ __device__ __forceinline__ double conv_flux(const FluxIn& F, int neq, int dir,
     case 3:
     case 4: {
       const double v = -F.UM(2 + dir, p) * F.U(neq - 1, p);
-      return (dir == neq - 2) ? v - F.pres[p] : v;
+      return (dir == neq - 2) ? v - pres_of(F, p) : v;
     }
     default:
       return -F.U(1 + dir, p) * F.UM(5, p);
@@ -103,9 +133,9 @@ __device__ __forceinline__ double conv_flux(const FluxIn& F, int neq, int dir,
 // Viscous flux of equation neq (2..4) along dir (0:ev, 1:fv, 2:gv),
 // solver/flujos_compact.f90:54-173.  The 3x3 table of (metric, dvel)
 // pairs is symmetric: entry (a,b) couples velocity a and direction b.
-__device__ __forceinline__ double visc_flux(const FluxIn& F, int neq, int dir,
-                                            long long p) {
-  const double mu = F.vis[p];
+template <class F_>
+__device__ __forceinline__ double visc_flux(const F_& F, int neq, int dir, long long p) {
+  const double mu = vis_of(F, p);
   const int a = neq - 2;  // velocity component 0..2
   if (a == dir) {
     // 2*mu*metric_a * d(u_a)/d(xi_a): dvel index 1, 5, 9
@@ -259,6 +289,7 @@ struct ProbDivAll {
 // materialized form of the flux divergence.
 struct ProbAccum {
   static constexpr int NV = 1;
+  static constexpr bool PREFETCH = true;
   static const char* label() { return "flux_div_all[accumulate]"; }
   double bytes_pt() const { return 8.0 * (2 + (pass > 0 ? 1 : 0) + (pass == 2 ? 1 : 0)); }
   const double* f;

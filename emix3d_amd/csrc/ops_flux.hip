@@ -27,21 +27,71 @@ __global__ void k_flujos(FluxIn F, int neq, double* flux) {
   }
 }
 
-// The eight projected fluxes J*(metric_dir*flux) of direction dir (slots
-// of ProbDivAll: convective 1..5, viscous 2..4), the arithmetic of
-// ProbDivAll::load.
-__global__ void k_flux_project(FluxIn F, int dir, double* out) {
+// The eight projected fluxes J*(metric_dir*flux) (slots of ProbDivAll:
+// convective 1..5, viscous 2..4) of every direction in MASK, the
+// arithmetic of ProbDivAll::load.  The inputs are read once per point for
+// all directions; block k of eight output fields holds the k-th direction
+// of MASK in ascending order.
+template <int MASK>
+__global__ void k_flux_project(FluxIn F, double* out) {
   const long long N = F.N;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
        p += (long long)gridDim.x * blockDim.x) {
-    const double det = F.jbn[9 * N + p];
-    const double met = F.jbn[(4 * dir) * N + p];
+    const FluxPoint P(F, p);
+    const double det = P.J(10, p);
+    int k = 0;
 #pragma unroll
-    for (int slot = 0; slot < 8; slot++) {
-      const double f = slot < 5 ? conv_flux(F, slot + 1, dir, p) : visc_flux(F, slot - 3, dir, p);
-      out[slot * N + p] = det * (met * f);
+    for (int dir = 0; dir < 3; dir++) {
+      if (!(MASK & (1 << dir))) continue;
+      const double met = P.J(1 + 4 * dir, p);
+#pragma unroll
+      for (int slot = 0; slot < 8; slot++) {
+        const double f = slot < 5 ? conv_flux(P, slot + 1, dir, p) : visc_flux(P, slot - 3, dir, p);
+        out[(8 * k + slot) * N + p] = det * (met * f);
+      }
+      k++;
     }
   }
+}
+
+// Distinct input fields of the projected fluxes of the directions in mask:
+// um(2+dir) and um5, u 1..3, pres, vis, J and the metrics 1/5/9, and the
+// dvel components of the viscous fluxes (diagonal 1+4a, off-diagonal pairs
+// (0,1): 2,4; (0,2): 3,7; (1,2): 6,8).
+static int project_reads(int mask) {
+  int um = 1, dv = 0;
+  for (int dir = 0; dir < 3; dir++) {
+    if (!(mask & (1 << dir))) continue;
+    um |= 2 << dir;
+    for (int a = 0; a < 3; a++) {
+      if (a == dir) { dv |= 1 << (4 * a); continue; }
+      const int lo = a < dir ? a : dir, hi = a < dir ? dir : a;
+      if (lo == 0 && hi == 1) dv |= (1 << 1) | (1 << 3);
+      else if (lo == 0 && hi == 2) dv |= (1 << 2) | (1 << 6);
+      else dv |= (1 << 7) | (1 << 5);
+    }
+  }
+  return __builtin_popcount(um) + 3 + 2 + 4 + __builtin_popcount(dv);
+}
+
+static int launch_project(emx_ctx* ctx, const FluxIn& in, int mask) {
+  const long long N = in.N;
+  const double bytes = 8.0 * N * (project_reads(mask) + 8 * __builtin_popcount(mask));
+  const dim3 g(pointwise_blocks(N)), b(256);
+  switch (mask) {
+#define EMX_PROJECT(M)                                                               \
+  case M:                                                                            \
+    EMX_TIMED(ctx, "flux_project", bytes,                                           \
+              hipLaunchKernelGGL(k_flux_project<M>, g, b, 0, ctx->stream, in, ctx->fluxbuf)); \
+    break;
+    EMX_PROJECT(1) EMX_PROJECT(2) EMX_PROJECT(3) EMX_PROJECT(4) EMX_PROJECT(5)
+    EMX_PROJECT(6) EMX_PROJECT(7)
+#undef EMX_PROJECT
+    default:
+      return EMX_OK;
+  }
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
 }
 
 int op_flux(emx_ctx* ctx, int neq, const FluxPtrs& fp, double* flux) {
@@ -101,33 +151,48 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
                     double* rs, double* rsv) {
   const long long N = ctx->d.N();
   const FluxIn in{fp.um, fp.u, fp.pres, fp.vis, fp.jbn, fp.dvel, N};
+  // Directions on the materialized kind go first: their projected fluxes
+  // come from one pass over the inputs, and each flux then takes one
+  // single-field solve on the derivative plan.  The fused directions follow,
+  // so that the last pass, which also reads 1/J, reads the inputs and 1/J
+  // once for all eight fluxes.  (The three directional terms are summed in
+  // pass order.)
+  int mat = 0;
+  for (int dir = 0; dir < 3; dir++)
+    if (ctx->plan[PLAN_FLUX][dir].kind == KIND_MATERIALIZED) mat |= 1 << dir;
+  if (__builtin_popcount(mat) > ctx->fluxbuf_dirs) {
+    set_error("flux divergence: projected-flux scratch holds %d directions, plan needs %d",
+              ctx->fluxbuf_dirs, __builtin_popcount(mat));
+    return EMX_ERR_NOMEM;
+  }
+  EMX_TRY(launch_project(ctx, in, mat));
+  int pass = 0, k = 0;
   for (int dir = 0; dir < 3; dir++) {
-    const DirPlan& pl = ctx->plan[PLAN_FLUX][dir];
-    if (pl.kind == KIND_MATERIALIZED) {
-      // projected fluxes into scratch (16 inputs read, 8 written), then one
-      // single-field solve per flux on the derivative plan
-      EMX_TIMED(ctx, "flux_project", 24.0 * 8.0 * N,
-                hipLaunchKernelGGL(k_flux_project, dim3(pointwise_blocks(N)), dim3(256), 0,
-                                   ctx->stream, in, dir, ctx->fluxbuf));
-      EMX_TRY_HIP(hipGetLastError());
-      for (int slot = 0; slot < 8; slot++) {
-        double* out = slot < 5 ? rs + slot * N : rsv + (slot - 4) * N;
-        ProbAccum p{ctx->fluxbuf + slot * N, ctx->acc + slot * N, out, fp.jbn + 10 * N, dir};
-        EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
-      }
-      continue;
+    if (!(mat & (1 << dir))) continue;
+    for (int slot = 0; slot < 8; slot++) {
+      double* out = slot < 5 ? rs + slot * N : rsv + (slot - 4) * N;
+      ProbAccum p{ctx->fluxbuf + (8 * k + slot) * N, ctx->acc + slot * N, out, fp.jbn + 10 * N,
+                  pass};
+      EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
     }
+    k++;
+    pass++;
+  }
+  for (int dir = 0; dir < 3; dir++) {
+    if (mat & (1 << dir)) continue;
+    const DirPlan& pl = ctx->plan[PLAN_FLUX][dir];
     // one launch of all eight solves on the wavefront kind with short
     // chunks (8*C formed values per work-item), else two launches of four
     if (pl.kind == KIND_WAVE && pl.C <= 4) {
-      ProbDivAll<2> p{in, dir, dir, ctx->acc, rs, rsv};
+      ProbDivAll<2> p{in, dir, pass, ctx->acc, rs, rsv};
       EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
     } else {
-      ProbDivAll<0> p0{in, dir, dir, ctx->acc, rs, rsv};
+      ProbDivAll<0> p0{in, dir, pass, ctx->acc, rs, rsv};
       EMX_TRY(launch_lines(ctx, dir, p0, ih[dir]));
-      ProbDivAll<1> p1{in, dir, dir, ctx->acc, rs, rsv};
+      ProbDivAll<1> p1{in, dir, pass, ctx->acc, rs, rsv};
       EMX_TRY(launch_lines(ctx, dir, p1, ih[dir]));
     }
+    pass++;
   }
   // zero viscous divergence of the continuity and scalar equations
   EMX_TIMED(ctx, "fill", 8.0 * N,

@@ -76,8 +76,12 @@ int emx_set_stream(emx_ctx* ctx, void* stream);
  * Valid for op 0: kind 0 C in {1,2,4,8,16}; kind 1 {8,16,32}; kind 2
  * {4,8,16}; kind 3 {4,8}; kind 4 {4,8}.  Op 1: kind 0 C <= 16, kinds 2 and
  * 3 C in {4,8}, kind 4 C in {2,4}, and kind 5 with C = 0: the fluxes of
- * the direction projected into scratch by one pointwise kernel, then one
- * single-field solve each on the derivative plan.  Used by the autotuner. */
+ * the direction projected into scratch (one pointwise kernel for all
+ * directions on kind 5, scratch of 8 fields per such direction, grown by
+ * this call; EMX_ERR_NOMEM and the plan unchanged if that fails), then one
+ * single-field solve each on the derivative plan.  Directions on kind 5 run
+ * first, so the sum of the three directional terms follows the plan's pass
+ * order.  Used by the autotuner. */
 int emx_set_plan(emx_ctx* ctx, int op, int dir, int kind, int C);
 int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C);
 
@@ -89,8 +93,11 @@ int emx_get_plan(const emx_ctx* ctx, int op, int dir, int* kind, int* C);
  * aggregates per (kernel label, direction), sorted by total time; entry i
  * gives the label, direction (-1 for pointwise kernels), launch count,
  * summed duration and summed algorithmic bytes (each field the kernel must
- * read or write once, 8 B per grid point). */
+ * read or write once, 8 B per grid point).  emx_timing_select restricts the
+ * recording to the launches of one kernel label (NULL: every launch), so a
+ * timed loop carries events around that kernel only. */
 int emx_timing_begin(emx_ctx* ctx);
+int emx_timing_select(emx_ctx* ctx, const char* label);
 int emx_timing_end(emx_ctx* ctx, int* n_entries);
 int emx_timing_entry(const emx_ctx* ctx, int i, const char** label, int* dir,
                      int* launches, double* total_ms, double* bytes);

@@ -179,7 +179,8 @@ def test_flujos_divergencia(oracle, scheme):
 
 
 @pytest.mark.parametrize("scheme", SCHEMES)
-@pytest.mark.parametrize("plan", ["default", "strided", "wave_x_strided16", "strided8"])
+@pytest.mark.parametrize("plan", ["default", "strided", "wave_x_strided16", "strided8",
+                                  "materialized_all", "materialized_x"])
 def test_flux_div_all_equations(oracle, scheme, plan):
     """The RUNGEK equation loop in three passes equals flujos+divergencia of
     each equation, for every grouping the launch shapes select."""
@@ -199,6 +200,14 @@ def test_flux_div_all_equations(oracle, scheme, plan):
         op.set_plan(0, 4, 4, op=1)
         op.set_plan(1, 4, 2, op=1)
         op.set_plan(2, 4, 4, op=1)
+    elif plan == "materialized_all":
+        for d in range(3):
+            op.set_plan(d, 5, 0, op=1)
+    elif plan == "materialized_x":
+        # xi first from scratch, then the fused eta and zeta passes
+        op.set_plan(0, 5, 0, op=1)
+        op.set_plan(1, 2, 4, op=1)
+        op.set_plan(2, 0, 2, op=1)
     ih = op.default_ih()
     D = {k: dev(z[k]) for k in ("um", "u", "pres", "jbn", "dvel")}
     rs, rsv = op.flujos_divergencia_all(D["um"], D["u"], D["pres"], dev(vis), D["jbn"],
@@ -383,3 +392,21 @@ def test_rk_update_sub_steps(oracle):
     for sub, u in ((0, None), (4, dev(um)), (2, None)):
         with pytest.raises(EmxError):
             op.rk_update(sub, dev(um0), u, dev(rs[0]), dev(rsv[0]), dt)
+
+
+def test_timing_select_records_one_label():
+    """Per-launch timing restricted to one kernel label records only that
+    label; without a label every launch is recorded."""
+    n = 40
+    op = Operator(n, n, n, "c6c")
+    f = torch.randn((n, n, n), dtype=torch.float64, device="cuda")
+    um = torch.randn((5, n, n, n), dtype=torch.float64, device="cuda")
+    op.timing_begin(label="deriv")
+    op.derivative(f, 1)
+    op.rk_update(1, um, None, um, um, 0.1)
+    ents = op.timing_end()
+    assert [e["label"] for e in ents] == ["deriv"] and ents[0]["launches"] == 1
+    op.timing_begin()
+    op.derivative(f, 1)
+    op.rk_update(1, um, None, um, um, 0.1)
+    assert sorted(e["label"] for e in op.timing_end()) == ["deriv", "rk_update"]
