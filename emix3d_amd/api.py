@@ -368,6 +368,48 @@ class Operator:
                   self._check(rs, 5, "rs"), self._check(rsv, 5, "rsv"), float(dt))
         return um1
 
+    def frontera_val(self, um1):
+        """Boundary values z, y, x of um1 in place (frontera_{z,y,x}_val)."""
+        self._stream()
+        _lib.call("emx_frontera_val", self._ctx, self._check(um1, 5, "um1"))
+        return um1
+
+    def varc_to_varnc(self, um1, mach, u=None, pres=None, temp=None, conc=None):
+        """Primitive variables from um1 (varc_to_varnc); returns u, pres,
+        temp (conc filled when given)."""
+        u = self._empty(3) if u is None else u
+        pres = self._empty() if pres is None else pres
+        temp = self._empty() if temp is None else temp
+        self._stream()
+        _lib.call("emx_varc_to_varnc", self._ctx, self._check(um1, 5, "um1"), float(mach),
+                  self._check(u, 3, "u"), self._check(pres), self._check(temp),
+                  None if conc is None else self._check(conc))
+        return u, pres, temp
+
+    def varnc_to_varc(self, u, pres, temp, mach, um0=None, pres0=None):
+        """Conservative variables from u, pres, temp (varnc_to_varc)."""
+        um0 = self._empty(5) if um0 is None else um0
+        self._stream()
+        _lib.call("emx_varnc_to_varc", self._ctx, self._check(u, 3, "u"), self._check(pres),
+                  self._check(temp), float(mach), self._check(um0, 5, "um0"),
+                  None if pres0 is None else self._check(pres0))
+        return um0
+
+    def rk_clip(self, um1, mask, mach, u, pres, temp, conc=None):
+        """RUNGEK velocity clip of um1 in place, primitive variables out."""
+        self._stream()
+        _lib.call("emx_rk_clip", self._ctx, self._check(um1, 5, "um1"), self._check(mask),
+                  float(mach), self._check(u, 3, "u"), self._check(pres), self._check(temp),
+                  None if conc is None else self._check(conc))
+        return um1
+
+    def rk_rebuild(self, um1, u, pres, mach):
+        """um1(1..4) rebuilt from pres and u, in place."""
+        self._stream()
+        _lib.call("emx_rk_rebuild", self._ctx, self._check(um1, 5, "um1"), self._check(u, 3, "u"),
+                  self._check(pres), float(mach))
+        return um1
+
     def advect_level_set(self, pres, jbn, sobject, dxyz, deltamin):
         """advectLevelSet: extrapolate pres into the solid (in place)."""
         self._stream()
@@ -398,6 +440,25 @@ class Operator:
     def stage_buffers(self):
         return {"dvel": self._empty(9), "dtemp": self._empty(3), "amut": self._empty(),
                 "vis": self._empty(), "rs": self._empty(5), "rsv": self._empty(5)}
+
+    def rk_substep(self, sub, um0, um, u, temp, pres, jbn, sgd, c4, dt, mach, mask,
+                   conc=None, sobject=None, dxyz=None, deltamin=None, work=None, um1=None):
+        """One RUNGEK sub-step on the device (solver/rungek_ico.f90:79-141,
+        143-197, 201-265): the spatial operator on (um, u, temp, pres), the
+        SSP-RK3 update into um1, boundary values z/y/x, the velocity clip
+        with mask, the pressure extrapolation into the solid when sobject is
+        given (advectLevelSet(0)), and um1 rebuilt from pres and u.  u, pres,
+        temp (and conc) are updated in place; returns um1.  The impeller
+        forcing (rushton) is not applied.  For sub-step 1 the reference
+        first copies um0 into um, so pass um = um0."""
+        w = self.rk_stage(um, u, temp, pres, jbn, sgd, c4, work=work)
+        um1 = self.rk_update(sub, um0, None if sub == 1 else um, w["rs"], w["rsv"], dt, um1=um1)
+        self.frontera_val(um1)
+        self.rk_clip(um1, mask, mach, u, pres, temp, conc)
+        if sobject is not None:
+            self.advect_level_set(pres, jbn, sobject, dxyz, deltamin)
+        self.rk_rebuild(um1, u, pres, mach)
+        return um1
 
 
 class Batch:

@@ -456,6 +456,63 @@ int emx_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const d
   return op_rk_update(ctx, sub, um1, um0, um, rs, rsv, dt);
 }
 
+int emx_frontera_val(emx_ctx* ctx, double* um1) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(um1);
+  return op_frontera_val(ctx, um1);
+}
+
+static int check_mach(double mach, const char* who) {
+  if (!(mach > 0.0)) {
+    set_error("%s: Mach number %g must be positive", who, mach);
+    return EMX_ERR_ARG;
+  }
+  return EMX_OK;
+}
+
+int emx_varc_to_varnc(emx_ctx* ctx, const double* um1, double mach, double* u, double* pres,
+                      double* temp, double* conc) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(um1);
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  CHECK_PTR(temp);
+  EMX_TRY(check_mach(mach, "emx_varc_to_varnc"));
+  return op_varc_to_varnc(ctx, um1, mach, u, pres, temp, conc);
+}
+
+int emx_varnc_to_varc(emx_ctx* ctx, const double* u, const double* pres, const double* temp,
+                      double mach, double* um0, double* pres0) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  CHECK_PTR(temp);
+  CHECK_PTR(um0);
+  EMX_TRY(check_mach(mach, "emx_varnc_to_varc"));
+  return op_varnc_to_varc(ctx, u, pres, temp, mach, um0, pres0);
+}
+
+int emx_rk_clip(emx_ctx* ctx, double* um1, const double* mask, double mach, double* u,
+                double* pres, double* temp, double* conc) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(um1);
+  CHECK_PTR(mask);
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  CHECK_PTR(temp);
+  EMX_TRY(check_mach(mach, "emx_rk_clip"));
+  return op_rk_clip(ctx, um1, mask, mach, u, pres, temp, conc);
+}
+
+int emx_rk_rebuild(emx_ctx* ctx, double* um1, const double* u, const double* pres, double mach) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(um1);
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  EMX_TRY(check_mach(mach, "emx_rk_rebuild"));
+  return op_rk_rebuild(ctx, um1, u, pres, mach);
+}
+
 int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
                 const double* delta, double* nut, double c4, double* vis) {
   EMX_TRY(check_ctx(ctx));

@@ -30,6 +30,14 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
 int op_metrics(emx_ctx* ctx, const double* x, double* jbn, int* bad_dev);
 int op_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const double* um,
                  const double* rs, const double* rsv, double dt);
+int op_frontera_val(emx_ctx* ctx, double* um1);
+int op_varc_to_varnc(emx_ctx* ctx, const double* um1, double mach, double* u, double* pres,
+                     double* temp, double* conc);
+int op_varnc_to_varc(emx_ctx* ctx, const double* u, const double* pres, const double* temp,
+                     double mach, double* um0, double* pres0);
+int op_rk_clip(emx_ctx* ctx, double* um1, const double* mask, double mach, double* u,
+               double* pres, double* temp, double* conc);
+int op_rk_rebuild(emx_ctx* ctx, double* um1, const double* u, const double* pres, double mach);
 int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
                     double* dmins, double* blockmax, int nblocks);
 int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,

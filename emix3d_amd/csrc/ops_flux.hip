@@ -205,34 +205,5 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
   return EMX_OK;
 }
 
-// SSP-RK3 update of the conservative variables after the flux divergence
-// of sub-step `sub` (solver/rungek_ico.f90:87-91, 152-153, 216-219), in the
-// reference's evaluation order:
-//   1: um1 = um0 + dt*(rs + rsv)
-//   2: um1 = (0.75*um0 + 0.25*um) + (0.25*dt)*(rs + rsv)
-//   3: um1 = ((1/3)*um0 + (2/3)*um) + ((2/3)*dt)*(rs + rsv)
-__global__ void k_rk_update(int sub, double* um1, const double* um0, const double* um,
-                           const double* rs, const double* rsv, double dt, long long n) {
-  const double third = 1.0 / 3.0, two3 = 2.0 / 3.0;
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (long long)gridDim.x * blockDim.x) {
-    const double r = rs[p] + rsv[p];
-    double v;
-    if (sub == 1) v = um0[p] + dt * r;
-    else if (sub == 2) v = 0.75 * um0[p] + 0.25 * um[p] + 0.25 * dt * r;
-    else v = third * um0[p] + two3 * um[p] + two3 * dt * r;
-    um1[p] = v;
-  }
-}
-
-int op_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const double* um,
-                 const double* rs, const double* rsv, double dt) {
-  const long long n = 5 * ctx->d.N();
-  EMX_TIMED(ctx, "rk_update", (sub == 1 ? 32.0 : 40.0) * n,
-            hipLaunchKernelGGL(k_rk_update, dim3(pointwise_blocks(n)), dim3(256), 0, ctx->stream,
-                               sub, um1, um0, um, rs, rsv, dt, n));
-  EMX_TRY_HIP(hipGetLastError());
-  return EMX_OK;
-}
 
 }  // namespace emx

@@ -285,11 +285,34 @@ int emx_flux_div_all_adj(emx_ctx* ctx, const double* rs_bar, const double* rsv_b
  * divergence of sub-step sub (1..3), solver/rungek_ico.f90:87-91, 152-153,
  * 216-219: um1 = um0 + dt*(rs+rsv); 0.75*um0 + 0.25*um + 0.25*dt*(rs+rsv);
  * um0/3 + 2*um/3 + (2/3)*dt*(rs+rsv).  All arrays 5 comps; um may be NULL
- * for sub-step 1.  (The boundary values, impeller forcing and the
- * primitive-variable update that follow in RUNGEK are not part of this
- * library.) */
+ * for sub-step 1.  (The impeller forcing that follows in RUNGEK, rushton of
+ * src/Geometry/rush_ruth_me.f90, is not part of this library.) */
 int emx_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const double* um,
                   const double* rs, const double* rsv, double dt);
+
+/* Boundary values of um1 (5 comps) in RUNGEK's order (rungek_ico.f90:96-98):
+ * frontera_z_val (src/MSVSingle/frontera_z.f90:219-262), frontera_y_val
+ * (frontera_y.f90:220-247), frontera_x_val (frontera_x.f90:223-256).  In
+ * place; three boundary-plane launches. */
+int emx_frontera_val(emx_ctx* ctx, double* um1);
+/* varc_to_varnc (src/MSVSingle/herramientas_incomp2.f90:2-30): temp (and
+ * conc, may be NULL) = um1(5), pres = um1(1)/(Mach*Mach), u(c) =
+ * um1(1+c)/um1(1).  mach > 0 (EMX_ERR_ARG otherwise). */
+int emx_varc_to_varnc(emx_ctx* ctx, const double* um1, double mach, double* u, double* pres,
+                      double* temp, double* conc);
+/* varnc_to_varc (herramientas_incomp2.f90:32-58): um0(1) = pres*Mach*Mach,
+ * um0(5) = temp, um0(1+c) = u(c)*um0(1); pres0 (may be NULL) = pres. */
+int emx_varnc_to_varc(emx_ctx* ctx, const double* u, const double* pres, const double* temp,
+                      double mach, double* um0, double* pres0);
+/* The velocity clip of RUNGEK (solver/rungek_ico.f90:102-123), fused per
+ * point: varc_to_varnc of um1, u clipped to [-6, 6], um1(1+c) =
+ * u(c)*um1(1)*mask, varc_to_varnc again into u, pres, temp (conc may be
+ * NULL).  um1 in place. */
+int emx_rk_clip(emx_ctx* ctx, double* um1, const double* mask, double mach, double* u,
+                double* pres, double* temp, double* conc);
+/* um1 rebuilt after the level-set step (rungek_ico.f90:126-135):
+ * um1(1) = pres*Mach*Mach, um1(1+c) = u(c)*um1(1); um1(5) untouched. */
+int emx_rk_rebuild(emx_ctx* ctx, double* um1, const double* u, const double* pres, double mach);
 
 /* Pressure extrapolation into the solid (sobject > 0): advect_CUDA
  * (src/LevelSet/lsTransportEc.cu:24-197 -> interface.cu:341-436).

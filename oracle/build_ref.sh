@@ -36,6 +36,10 @@ for scheme in c4 c6f; do
     "$REF/solver/sgdm_incomp.f90"
     "$REF/solver/viscosidad_incomp.f90"
     "$REF/solver/jacob_malla.f90"
+    "$REF/src/MSVSingle/herramientas_incomp2.f90"
+    "$REF/src/MSVSingle/filtro_incomp.f90"
+    "$REF/src/MSVSingle/frontera_y.f90"
+    "$REF/src/MSVSingle/frontera_z.f90"
     "$HERE/ref_driver.f90"
   )
   OBJS=()

@@ -1278,3 +1278,110 @@ int orc_rk_update(int sub, long n, double *um1, const double *um0, const double 
     }
     return 0;
 }
+
+
+/* ---- RUNGEK glue around the operator ------------------------------------ */
+
+#define UM1(i, j, k, m) um1[(i) + (long)nx * ((j) + (long)ny * ((k) + (long)nz * (m)))]
+
+/* frontera_z_val, frontera_y_val, frontera_x_val (cited in the header),
+ * statement by statement per boundary column so overlapping edges see the
+ * same order as the Fortran loops. */
+int orc_frontera_val(int nx, int ny, int nz, double *um1)
+{
+    if (nx < 3 || ny < 3 || nz < 3) return -3;
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            UM1(i, j, 0, 0) = UM1(i, j, 1, 0);
+            UM1(i, j, 0, 1) = UM1(i, j, 1, 1);
+            UM1(i, j, 0, 2) = UM1(i, j, 1, 2);
+            UM1(i, j, 0, 3) = 0.0;
+            UM1(i, j, 0, 4) = UM1(i, j, 1, 4);
+            UM1(i, j, nz - 1, 0) = 1.0;
+            UM1(i, j, nz - 2, 0) = 1.0;
+            UM1(i, j, nz - 1, 1) = UM1(i, j, nz - 3, 1);
+            UM1(i, j, nz - 2, 1) = UM1(i, j, nz - 3, 1);
+            UM1(i, j, nz - 1, 2) = UM1(i, j, nz - 3, 2);
+            UM1(i, j, nz - 2, 2) = UM1(i, j, nz - 3, 2);
+            UM1(i, j, nz - 1, 3) = 0.0;
+            UM1(i, j, nz - 2, 3) = 0.0;
+            UM1(i, j, nz - 1, 4) = UM1(i, j, nz - 3, 4);
+            UM1(i, j, nz - 2, 4) = UM1(i, j, nz - 3, 4);
+        }
+    for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nx; i++) {
+            UM1(i, 0, k, 0) = UM1(i, 1, k, 0);
+            UM1(i, ny - 1, k, 0) = UM1(i, ny - 2, k, 0);
+            UM1(i, 0, k, 1) = 0.0;
+            UM1(i, ny - 1, k, 1) = 0.0;
+            UM1(i, 0, k, 2) = 0.0;
+            UM1(i, ny - 1, k, 2) = 0.0;
+            UM1(i, 0, k, 3) = 0.0;
+            UM1(i, ny - 1, k, 3) = 0.0;
+            UM1(i, 0, k, 4) = UM1(i, 1, k, 4);
+            UM1(i, ny - 1, k, 4) = UM1(i, ny - 2, k, 4);
+        }
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++) {
+            UM1(0, j, k, 1) = 0.0;
+            UM1(0, j, k, 2) = 0.0;
+            UM1(0, j, k, 3) = 0.0;
+            UM1(0, j, k, 4) = UM1(1, j, k, 4);
+            UM1(0, j, k, 0) = UM1(1, j, k, 0);
+            UM1(nx - 1, j, k, 1) = 0.0;
+            UM1(nx - 1, j, k, 2) = 0.0;
+            UM1(nx - 1, j, k, 3) = 0.0;
+            UM1(nx - 1, j, k, 4) = UM1(nx - 2, j, k, 4);
+            UM1(nx - 1, j, k, 0) = UM1(nx - 2, j, k, 0);
+        }
+    return 0;
+}
+#undef UM1
+
+int orc_varc_to_varnc(long n, const double *um1, double mach, double *u, double *pres,
+                      double *temp, double *conc)
+{
+    for (long p = 0; p < n; p++) {
+        if (conc) conc[p] = um1[4 * n + p];
+        temp[p] = um1[4 * n + p];
+        pres[p] = um1[p] / (mach * mach);
+        for (int c = 0; c < 3; c++) u[c * n + p] = um1[(1 + c) * n + p] / um1[p];
+    }
+    return 0;
+}
+
+int orc_varnc_to_varc(long n, const double *u, const double *pres, const double *temp,
+                      double mach, double *um0, double *pres0)
+{
+    for (long p = 0; p < n; p++) {
+        um0[p] = pres[p] * mach * mach;
+        um0[4 * n + p] = temp[p];
+        for (int c = 0; c < 3; c++) um0[(1 + c) * n + p] = u[c * n + p] * um0[p];
+        if (pres0) pres0[p] = pres[p];
+    }
+    return 0;
+}
+
+int orc_rk_clip(long n, double *um1, const double *mask, double mach, double *u, double *pres,
+                double *temp, double *conc)
+{
+    orc_varc_to_varnc(n, um1, mach, u, pres, temp, conc);
+    for (long p = 0; p < n; p++)
+        for (int c = 0; c < 3; c++) {
+            double v = u[c * n + p];
+            v = v < 6.0 ? v : 6.0;     /* min(6, u) */
+            v = v > -6.0 ? v : -6.0;   /* max(-6, u) */
+            u[c * n + p] = v;
+            um1[(1 + c) * n + p] = v * um1[p] * mask[p];
+        }
+    return orc_varc_to_varnc(n, um1, mach, u, pres, temp, conc);
+}
+
+int orc_rk_rebuild(long n, double *um1, const double *u, const double *pres, double mach)
+{
+    for (long p = 0; p < n; p++) {
+        um1[p] = pres[p] * mach * mach;
+        for (int c = 0; c < 3; c++) um1[(1 + c) * n + p] = u[c * n + p] * um1[p];
+    }
+    return 0;
+}

@@ -132,6 +132,26 @@ int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gphys,
 int orc_rk_update(int sub, long n, double *um1, const double *um0, const double *um,
                   const double *rs, const double *rsv, double dt);
 
+/* Boundary values of um1 (nx,ny,nz,5) in the RUNGEK order z, y, x:
+ * frontera_z_val (src/MSVSingle/frontera_z.f90:219-262), frontera_y_val
+ * (frontera_y.f90:220-247), frontera_x_val (frontera_x.f90:223-256). */
+int orc_frontera_val(int nx, int ny, int nz, double *um1);
+/* varc_to_varnc (src/MSVSingle/herramientas_incomp2.f90:2-30): conc and
+ * temp = um1(5), pres = um1(1)/(Mach*Mach), u(c) = um1(1+c)/um1(1); conc may
+ * be NULL. */
+int orc_varc_to_varnc(long n, const double *um1, double mach, double *u, double *pres,
+                      double *temp, double *conc);
+/* varnc_to_varc (herramientas_incomp2.f90:32-58); pres0 may be NULL. */
+int orc_varnc_to_varc(long n, const double *u, const double *pres, const double *temp,
+                      double mach, double *um0, double *pres0);
+/* The RUNGEK velocity clip (solver/rungek_ico.f90:102-123): varc_to_varnc,
+ * u clipped to [-6, 6], um1(2..4) = u*um1(1)*mask, varc_to_varnc again. */
+int orc_rk_clip(long n, double *um1, const double *mask, double mach, double *u, double *pres,
+                double *temp, double *conc);
+/* The rebuild after the level-set step (rungek_ico.f90:126-135):
+ * um1(1) = pres*Mach*Mach, um1(1+c) = u(c)*um1(1). */
+int orc_rk_rebuild(long n, double *um1, const double *u, const double *pres, double mach);
+
 /* ---- learned closure network (no reference counterpart) ------------------ */
 /* 10 -> 64 -> 64 -> 64 -> 1, SiLU hidden activations, softplus output.
  * params (9089): W1[64][10] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]

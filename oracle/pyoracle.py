@@ -62,6 +62,11 @@ def lib():
             "orc_weno5_derivs": [_I, _I, _I, _P, _P, _D, _D, _D, _P, _P],
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
             "orc_rk_update": [_I, _L, _P, _P, _P, _P, _P, _D],
+            "orc_frontera_val": [_I, _I, _I, _P],
+            "orc_varc_to_varnc": [_L, _P, _D, _P, _P, _P, _P],
+            "orc_varnc_to_varc": [_L, _P, _P, _P, _D, _P, _P],
+            "orc_rk_clip": [_L, _P, _P, _D, _P, _P, _P, _P],
+            "orc_rk_rebuild": [_L, _P, _P, _P, _D],
             "orc_mlp_forward": [_P, _P, _P, _L, _P],
             "orc_mlp_grad": [_P, _P, _P, _P, _L, _P],
             "orc_mlp_features": [_I, _I, _I, _P, _P, _P, _P],
@@ -180,6 +185,54 @@ def rk_update(sub, um0, um, rs, rsv, dt):
                                _ptr(np.ascontiguousarray(rs, dtype=np.float64)),
                                _ptr(np.ascontiguousarray(rsv, dtype=np.float64)), dt), "rk_update")
     return out
+
+
+def frontera_val(um1):
+    """Boundary values z, y, x on a copy of um1 (5,nz,ny,nx)."""
+    out = np.array(um1, dtype=np.float64, copy=True)
+    nz, ny, nx = out.shape[1:]
+    _check(lib().orc_frontera_val(nx, ny, nz, _ptr(out)), "frontera_val")
+    return out
+
+
+def varc_to_varnc(um1, mach):
+    um1 = np.ascontiguousarray(um1, dtype=np.float64)
+    n = um1[0].size
+    u = np.empty((3,) + um1.shape[1:])
+    pres, temp, conc = (np.empty(um1.shape[1:]) for _ in range(3))
+    _check(lib().orc_varc_to_varnc(n, _ptr(um1), mach, _ptr(u), _ptr(pres), _ptr(temp),
+                                   _ptr(conc)), "varc_to_varnc")
+    return u, pres, temp, conc
+
+
+def varnc_to_varc(u, pres, temp, mach):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    n = u[0].size
+    um0 = np.empty((5,) + u.shape[1:])
+    pres0 = np.empty(u.shape[1:])
+    _check(lib().orc_varnc_to_varc(n, _ptr(u), _ptr(np.ascontiguousarray(pres)),
+                                   _ptr(np.ascontiguousarray(temp)), mach, _ptr(um0),
+                                   _ptr(pres0)), "varnc_to_varc")
+    return um0, pres0
+
+
+def rk_clip(um1, mask, mach):
+    """Returns (um1 after the clip, u, pres, temp, conc)."""
+    um1 = np.array(um1, dtype=np.float64, copy=True)
+    n = um1[0].size
+    u = np.empty((3,) + um1.shape[1:])
+    pres, temp, conc = (np.empty(um1.shape[1:]) for _ in range(3))
+    _check(lib().orc_rk_clip(n, _ptr(um1), _ptr(np.ascontiguousarray(mask, dtype=np.float64)),
+                             mach, _ptr(u), _ptr(pres), _ptr(temp), _ptr(conc)), "rk_clip")
+    return um1, u, pres, temp, conc
+
+
+def rk_rebuild(um1, u, pres, mach):
+    um1 = np.array(um1, dtype=np.float64, copy=True)
+    n = um1[0].size
+    _check(lib().orc_rk_rebuild(n, _ptr(um1), _ptr(np.ascontiguousarray(u)),
+                                _ptr(np.ascontiguousarray(pres)), mach), "rk_rebuild")
+    return um1
 
 
 def _f32(a):
@@ -350,6 +403,30 @@ class RefFortran:
         line = np.array(line, dtype=np.float64, copy=True)
         self.L.ref_derivnper(_I(line.size), _ptr(line), _I(idir))
         return line
+
+    def frontera_zy(self, um1):
+        """frontera_z_val then frontera_y_val on a copy of um1 (5,nz,ny,nx)."""
+        self.init(*dims(um1))
+        out = np.array(um1, dtype=np.float64, copy=True)
+        self.L.ref_frontera_zy(_ptr(out))
+        return out
+
+    def varc_to_varnc(self, um1, mach):
+        self.init(*dims(um1))
+        u = np.empty((3,) + um1.shape[1:])
+        pres, temp, conc = (np.empty(um1.shape[1:]) for _ in range(3))
+        self.L.ref_varc_to_varnc(_ptr(np.ascontiguousarray(um1)), ctypes.c_double(mach),
+                                 _ptr(u), _ptr(pres), _ptr(temp), _ptr(conc))
+        return u, pres, temp, conc
+
+    def varnc_to_varc(self, u, pres, temp, mach):
+        self.init(*dims(u))
+        um0 = np.empty((5,) + u.shape[1:])
+        pres0 = np.empty(u.shape[1:])
+        self.L.ref_varnc_to_varc(_ptr(np.ascontiguousarray(u)), _ptr(np.ascontiguousarray(pres)),
+                                 _ptr(np.ascontiguousarray(temp)), ctypes.c_double(mach),
+                                 _ptr(um0), _ptr(pres0))
+        return um0, pres0
 
     def jacobeano(self, x):
         self.init(*dims(x))

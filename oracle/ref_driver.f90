@@ -360,3 +360,72 @@ end subroutine ref_stage_fetch
 subroutine ref_flush() bind(C, name='ref_flush')
   flush(6)
 end subroutine ref_flush
+
+! ---- RUNGEK glue: boundary values and variable conversions -----------------
+
+! frontera_z_val then frontera_y_val on um1, RUNGEK's first two boundary
+! calls (solver/rungek_ico.f90:96-97).  frontera_x_val is not built: its
+! source file also holds a routine that needs Intel's ifport module.
+subroutine ref_frontera_zy(umio) bind(C, name='ref_frontera_zy')
+  use iso_c_binding
+  use dimensiones
+  use variables
+  implicit none
+  real(c_double), intent(inout) :: umio(nx,ny,nz,5)
+  if (allocated(um1)) deallocate(um1)
+  allocate(um1(nx,ny,nz,5))
+  um1 = umio
+  call frontera_z_val()
+  call frontera_y_val()
+  umio = um1
+  deallocate(um1)
+end subroutine ref_frontera_zy
+
+subroutine ref_varc_to_varnc(umin, machin, uout, pout, tout, cout) &
+    bind(C, name='ref_varc_to_varnc')
+  use iso_c_binding
+  use dimensiones
+  use variables
+  use velocidades
+  use flow
+  implicit none
+  real(c_double), intent(in) :: umin(nx,ny,nz,5)
+  real(c_double), value :: machin
+  real(c_double), intent(out) :: uout(nx,ny,nz,3), pout(nx,ny,nz), tout(nx,ny,nz)
+  real(c_double), intent(out) :: cout(nx,ny,nz)
+  if (allocated(um1)) deallocate(um1)
+  allocate(um1(nx,ny,nz,5))
+  if (.not. allocated(conc)) allocate(conc(nx,ny,nz))
+  um1 = umin
+  mach = machin
+  call varc_to_varnc()
+  uout = u
+  pout = pres
+  tout = temp
+  cout = conc
+  deallocate(um1)
+end subroutine ref_varc_to_varnc
+
+subroutine ref_varnc_to_varc(uin, pin, tin, machin, umout, p0out) &
+    bind(C, name='ref_varnc_to_varc')
+  use iso_c_binding
+  use dimensiones
+  use variables
+  use velocidades
+  use flow
+  implicit none
+  real(c_double), intent(in) :: uin(nx,ny,nz,3), pin(nx,ny,nz), tin(nx,ny,nz)
+  real(c_double), value :: machin
+  real(c_double), intent(out) :: umout(nx,ny,nz,5), p0out(nx,ny,nz)
+  if (allocated(um0)) deallocate(um0)
+  allocate(um0(nx,ny,nz,5))
+  if (.not. allocated(pres0)) allocate(pres0(nx,ny,nz))
+  u = uin
+  pres = pin
+  temp = tin
+  mach = machin
+  call varnc_to_varc()
+  umout = um0
+  p0out = pres0
+  deallocate(um0)
+end subroutine ref_varnc_to_varc

@@ -5,7 +5,8 @@ oracle/build_ref.sh into oracle/_ref/) on seeded inputs and stores inputs
 and outputs as .npz fixtures next to this script.  Run in the build
 container only (the reference tree is not shipped):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # every fixture
+    python tests/golden/make_golden.py glue     # the RUNGEK glue fixture only
 """
 import os
 import sys
@@ -67,7 +68,33 @@ def make_grid(scheme, name, shape, curv, turb, seed):
     return out
 
 
+def make_glue():
+    """RUNGEK glue of the reference (frontera_z_val + frontera_y_val,
+    varc_to_varnc, varnc_to_varc) on a seeded conservative state."""
+    ref = RefFortran("c6f")
+    nx, ny, nz = 9, 8, 7
+    ref.init(nx, ny, nz)
+    rng = np.random.default_rng(20261211)
+    um1 = rng.standard_normal((5, nz, ny, nx))
+    um1[0] = 1.0 + 0.2 * rng.random((nz, ny, nx))
+    mach = 0.3
+    out = dict(um1=um1, mach=np.array(mach))
+    out["frontera_zy"] = ref.frontera_zy(um1)
+    out["u"], out["pres"], out["temp"], out["conc"] = ref.varc_to_varnc(um1, mach)
+    uin = rng.standard_normal((3, nz, ny, nx))
+    pin = 1.0 + rng.random((nz, ny, nx))
+    tin = rng.standard_normal((nz, ny, nx))
+    out.update(uin=uin, pin=pin, tin=tin)
+    out["um0"], out["pres0"] = ref.varnc_to_varc(uin, pin, tin, mach)
+    return out
+
+
 def main():
+    if sys.argv[1:] == ["glue"]:
+        np.savez_compressed(os.path.join(HERE, "ref_glue_g9x8x7.npz"), **make_glue())
+        print("glue vectors written to", HERE)
+        return
+    np.savez_compressed(os.path.join(HERE, "ref_glue_g9x8x7.npz"), **make_glue())
     for scheme in ("c4", "c6f"):
         if not RefFortran.available(scheme):
             raise SystemExit("oracle/_ref is not built: run `make -C oracle ref`")
