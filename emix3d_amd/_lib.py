@@ -47,6 +47,7 @@ SIGNATURES = {
     "emx_varnc_to_varc": ([_P, _P, _P, _P, _D, _P, _P], _I),
     "emx_rk_clip": ([_P, _P, _P, _D, _P, _P, _P, _P], _I),
     "emx_rk_rebuild": ([_P, _P, _P, _P, _D], _I),
+    "emx_filtrado": ([_P, _I, _P, _P, _D, _D, _P, _P, _P], _I),
     "emx_grad_div": ([_P, _P, _P, _P, _P, _D, _D, _D], _I),
     "emx_batch_create": ([ctypes.POINTER(_P), _I, ctypes.POINTER(_I), _I, _P], _I),
     "emx_batch_destroy": ([_P], _I),

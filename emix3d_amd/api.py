@@ -410,6 +410,18 @@ class Operator:
                   self._check(pres), float(mach))
         return um1
 
+    def filtrado(self, itf, um1, jbn, cfilt, mach, u=None, pres=None, temp=None):
+        """Compact filter FILTRADO(itf) of um1 in place; returns u, pres,
+        temp recomputed from the filtered variables."""
+        u = self._empty(3) if u is None else u
+        pres = self._empty() if pres is None else pres
+        temp = self._empty() if temp is None else temp
+        self._stream()
+        _lib.call("emx_filtrado", self._ctx, int(itf), self._check(um1, 5, "um1"),
+                  self._check(jbn, 11, "jbn"), float(cfilt), float(mach), self._check(u, 3, "u"),
+                  self._check(pres), self._check(temp))
+        return u, pres, temp
+
     def advect_level_set(self, pres, jbn, sobject, dxyz, deltamin):
         """advectLevelSet: extrapolate pres into the solid (in place)."""
         self._stream()

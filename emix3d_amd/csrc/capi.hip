@@ -513,6 +513,26 @@ int emx_rk_rebuild(emx_ctx* ctx, double* um1, const double* u, const double* pre
   return op_rk_rebuild(ctx, um1, u, pres, mach);
 }
 
+int emx_filtrado(emx_ctx* ctx, int itf, double* um1, const double* jbn, double cfilt,
+                 double mach, double* u, double* pres, double* temp) {
+  EMX_TRY(check_ctx(ctx));
+  if (itf < 1 || itf > 6) {
+    set_error("emx_filtrado: direction order itf %d outside 1..6", itf);
+    return EMX_ERR_ARG;
+  }
+  if (!(cfilt >= 0.0 && cfilt < 0.5)) {
+    set_error("emx_filtrado: cfilt %g outside [0, 0.5)", cfilt);
+    return EMX_ERR_ARG;
+  }
+  CHECK_PTR(um1);
+  CHECK_PTR(jbn);
+  CHECK_PTR(u);
+  CHECK_PTR(pres);
+  CHECK_PTR(temp);
+  EMX_TRY(check_mach(mach, "emx_filtrado"));
+  return op_filtrado(ctx, itf, um1, jbn, cfilt, mach, u, pres, temp);
+}
+
 int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
                 const double* delta, double* nut, double c4, double* vis) {
   EMX_TRY(check_ctx(ctx));

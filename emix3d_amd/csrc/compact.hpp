@@ -66,6 +66,35 @@ __host__ __device__ __forceinline__ double rhs_row(const double* w, int l, int n
   }
 }
 
+// Compact-filter RHS of row l (FILTRONPER, src/MSVSingle/filtro_incomp.f90:
+// 318-337, constants :140-145): w[d] = u_{l+d}, d in [-5,5]; 0-based rows
+// 5..n-6 take the 11-point explicit filter, the five rows at each end u_l.
+// cfilt is the filter parameter; sums left to right as the Fortran.
+__host__ __device__ __forceinline__ double filter_rhs_row(const double* w, int l, int n,
+                                                          double cfilt) {
+  if (l < 5 || l > n - 6) return w[0];
+  const double arsf = (193. + 126. * cfilt) / 256.;
+  const double brsf = (105. + 302. * cfilt) / 256.;
+  const double crsf = (-15. + 30. * cfilt) / 64.;
+  const double drsf = (45. - 90. * cfilt) / 512.;
+  const double ersf = (-5. + 10. * cfilt) / 256.;
+  const double frsf = (1. - 2 * cfilt) / 512.;
+  return 0.5 * (arsf * (w[0] + w[0]) + brsf * (w[1] + w[-1]) + crsf * (w[2] + w[-2]) +
+                drsf * (w[3] + w[-3]) + ersf * (w[4] + w[-4]) + frsf * (w[5] + w[-5]));
+}
+
+// Right-hand side of line scheme S at row l; for the filter schemes the
+// last argument is cfilt, for the derivatives the inverse spacing.
+template <int S>
+__host__ __device__ __forceinline__ double line_rhs(const double* w, int l, int n, double p) {
+  if constexpr (S == FILTER_NP || S == FILTER_P) return filter_rhs_row(w, l, n, p);
+  else return rhs_row<S>(w, l, n, p);
+}
+
+// Half-width of the right-hand-side stencil of line scheme S.
+template <int S>
+struct scheme_halo { static constexpr int value = (S == FILTER_NP || S == FILTER_P) ? 5 : 3; };
+
 // Problem traits: TILE (many inputs per point, formed once), PREFETCH
 // (load() is plain loads of few fields, so the wavefront kind may keep the
 // next line's loads in flight while it solves the current one).
@@ -131,7 +160,8 @@ __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab&
                                                 long long blk, long long nblk) {
   constexpr int NV = Prob::NV;
   constexpr bool PRE = prefetches<Prob>::value;
-  constexpr int W = C + 6;
+  constexpr int H = scheme_halo<S>::value;
+  constexpr int W = C + 2 * H;
   constexpr int STR = C + 1;  // chunk stride in the transpose buffer
   constexpr int TS = 64 * C;  // per-point table stride (chunk order)
   extern __shared__ double lds[];
@@ -199,7 +229,7 @@ __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab&
       double w[W];
 #pragma unroll
       for (int q = 0; q < W; q++) {
-        const int l = clampi(s - 3 + q, 0, n - 1);
+        const int l = clampi(s - H + q, 0, n - 1);
         w[q] = xb[l + l / C];
       }
       // local forward sweep from a zero carry
@@ -209,7 +239,7 @@ __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab&
       for (int q = 0; q < C; q++) {
         const int l = s + q;
         if (act && l < n) {
-          const double r = rhs_row<S>(&w[q + 3], l, n, ih);
+          const double r = line_rhs<S>(&w[q + H], l, n, ih);
           yp = r * s_ib[q * 64 + lane] - s_m[q * 64 + lane] * yp;
           y[q] = yp;
         } else {
@@ -317,7 +347,8 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
                                                    int nlanes, long long si, long long so,
                                                    long long sl, double ih, int bx, int by) {
   constexpr int NV = Prob::NV;
-  constexpr int W = C + 6;
+  constexpr int H = scheme_halo<S>::value;
+  constexpr int W = C + 2 * H;
   constexpr int PMAX = 1024 / LG;  // most chunks a workgroup holds
   extern __shared__ double lds[];
   const int n = T.n, P = T.P, K = T.K, KB = T.KB;
@@ -344,7 +375,7 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
   double w[NV][W];
 #pragma unroll
   for (int q = 0; q < W; q++) {
-    const int l = clampi(cc * C - 3 + q, 0, n - 1);
+    const int l = clampi(cc * C - H + q, 0, n - 1);
     double v[NV];
     prob.load(base + (long long)l * sl, v);
 #pragma unroll
@@ -369,7 +400,7 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
       const double ibl = s_ib[l], ml = s_m[l];
 #pragma unroll
       for (int a = 0; a < NV; a++) {
-        const double r = rhs_row<S>(&w[a][q + 3], l, n, ih);
+        const double r = line_rhs<S>(&w[a][q + H], l, n, ih);
         yp[a] = r * ibl - ml * yp[a];
         y[a][q] = yp[a];
       }

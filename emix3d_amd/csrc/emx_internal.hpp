@@ -94,12 +94,20 @@ struct LineTab {
 // transposed (adjoint) derivative.
 constexpr int TAB_TRANSPOSED = 16;
 
+// Line "schemes" of the compact filter (not public scheme ids): the
+// tridiagonal matrix of FILTRONPER with INIFILTRO's non-periodic arrays
+// (xi, eta) or periodic arrays (zeta), and its 11-point right-hand side.
+// Their tables depend on cfilt as well.
+constexpr int FILTER_NP = 3;
+constexpr int FILTER_P = 4;
+
 // Builds (once per device) and returns the table.  Thread-safe.  scheme
-// may carry TAB_TRANSPOSED.
-int get_line_tab(int scheme, int n, int C, int kind, LineTab* out);
+// may carry TAB_TRANSPOSED; cfilt is the filter parameter of the filter
+// schemes (ignored otherwise).
+int get_line_tab(int scheme, int n, int C, int kind, LineTab* out, double cfilt = 0.0);
 // Host copy of the Thomas factors (for tests and the adjoint).
 int host_thomas(int scheme, int n, std::vector<double>& a,
-                std::vector<double>& b, std::vector<double>& c);
+                std::vector<double>& b, std::vector<double>& c, double cfilt = 0.0);
 int scheme_min_n(int scheme);
 
 // Launch configuration of one direction of one context.

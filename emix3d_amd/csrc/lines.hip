@@ -39,14 +39,20 @@ int scheme_min_n(int scheme) {
 //  C4 : solver/derivada.f90:10-62
 //  C6F: src/MSVSingle/derivada_incomp.f90:10-62
 //  C6C: src/Schemes/derivCompact.cu:165-189
+//  filter: src/MSVSingle/filtro_incomp.f90:30-139 (INIFILTRO sets cfilt2..5
+//  to zero on both of its branches, so the non-periodic arrays are zero on
+//  the five rows at each end and cfilt on rows 6..n-5, 1-based)
 int host_thomas(int scheme, int n, std::vector<double>& a,
-                std::vector<double>& b, std::vector<double>& c) {
-  if (scheme < 0 || scheme > 2) return EMX_ERR_ARG;
+                std::vector<double>& b, std::vector<double>& c, double cfilt) {
+  if (scheme < 0 || scheme > FILTER_P) return EMX_ERR_ARG;
   if (n < scheme_min_n(scheme)) return EMX_ERR_SIZE;
   a.assign(n, 0.0);
   b.assign(n, 1.0);
   c.assign(n, 0.0);
-  if (scheme == EMX_SCHEME_C4) {
+  if (scheme == FILTER_NP || scheme == FILTER_P) {
+    for (int l = 0; l < n; l++)
+      if (scheme == FILTER_P || (l + 1 >= 6 && l + 1 <= n - 5)) a[l] = c[l] = cfilt;
+  } else if (scheme == EMX_SCHEME_C4) {
     for (int l = 1; l < n - 1; l++) a[l] = c[l] = 1. / 4.;
     a[0] = 0.0; a[n - 1] = 3.0;
     c[0] = 3.0; c[n - 1] = 0.0;
@@ -63,9 +69,10 @@ namespace {
 
 struct TabKey {
   int dev, scheme, n, C, kind;
+  uint64_t aux;  // bits of cfilt (filter schemes)
   bool operator<(const TabKey& o) const {
-    return std::tie(dev, scheme, n, C, kind) <
-           std::tie(o.dev, o.scheme, o.n, o.C, o.kind);
+    return std::tie(dev, scheme, n, C, kind, aux) <
+           std::tie(o.dev, o.scheme, o.n, o.C, o.kind, o.aux);
   }
 };
 
@@ -87,11 +94,12 @@ void rhs_transposed(int n, std::vector<double>& BT) {
     }
 }
 
-int build_tab(int scheme_flags, int n, int C, int kind, LineTab* t) {
+int build_tab(int scheme_flags, int n, int C, int kind, LineTab* t, double cfilt) {
   const int scheme = scheme_flags & ~TAB_TRANSPOSED;
   const bool transposed = (scheme_flags & TAB_TRANSPOSED) != 0;
+  if (transposed && scheme > EMX_SCHEME_C6C) return EMX_ERR_ARG;
   std::vector<double> a, b, c;
-  EMX_TRY(host_thomas(scheme, n, a, b, c));
+  EMX_TRY(host_thomas(scheme, n, a, b, c, cfilt));
   std::vector<double> BT;
   if (transposed) {
     // row l of A^T: A[l-1][l], A[l][l], A[l+1][l] = c_{l-1}, b_l, a_{l+1}
@@ -242,18 +250,20 @@ int build_tab(int scheme_flags, int n, int C, int kind, LineTab* t) {
 
 }  // namespace
 
-int get_line_tab(int scheme, int n, int C, int kind, LineTab* out) {
+int get_line_tab(int scheme, int n, int C, int kind, LineTab* out, double cfilt) {
   int dev = 0;
   EMX_TRY_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_tab_mu);
-  TabKey key{dev, scheme, n, C, kind};
+  uint64_t bits = 0;
+  if ((scheme & ~TAB_TRANSPOSED) >= FILTER_NP) memcpy(&bits, &cfilt, sizeof bits);
+  TabKey key{dev, scheme, n, C, kind, bits};
   auto it = g_tabs.find(key);
   if (it != g_tabs.end()) {
     *out = it->second;
     return EMX_OK;
   }
   LineTab t{};
-  EMX_TRY(build_tab(scheme, n, C, kind, &t));
+  EMX_TRY(build_tab(scheme, n, C, kind, &t, cfilt));
   g_tabs[key] = t;
   *out = t;
   return EMX_OK;

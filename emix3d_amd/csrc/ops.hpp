@@ -38,6 +38,8 @@ int op_varnc_to_varc(emx_ctx* ctx, const double* u, const double* pres, const do
 int op_rk_clip(emx_ctx* ctx, double* um1, const double* mask, double mach, double* u,
                double* pres, double* temp, double* conc);
 int op_rk_rebuild(emx_ctx* ctx, double* um1, const double* u, const double* pres, double mach);
+int op_filtrado(emx_ctx* ctx, int itf, double* um1, const double* jbn, double cfilt,
+                double mach, double* u, double* pres, double* temp);
 int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
                     double* dmins, double* blockmax, int nblocks);
 int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,

@@ -314,6 +314,22 @@ int emx_rk_clip(emx_ctx* ctx, double* um1, const double* mask, double mach, doub
  * um1(1) = pres*Mach*Mach, um1(1+c) = u(c)*um1(1); um1(5) untouched. */
 int emx_rk_rebuild(emx_ctx* ctx, double* um1, const double* u, const double* pres, double mach);
 
+/* Compact filter FILTRADO(itf) (src/MSVSingle/herramientas_incomp2.f90:
+ * 227-410) with the arrays INIFILTRO builds for cfilt
+ * (src/MSVSingle/filtro_incomp.f90:2-166): each of the five fields of um1,
+ * times J (jbn 10), is filtered along the three directions in the order
+ * itf (1..6) selects -- 1: xi,eta,zeta  2: xi,zeta,eta  3: eta,xi,zeta
+ * 4: eta,zeta,xi  5: zeta,xi,eta  6: zeta,eta,xi -- one tridiagonal solve
+ * per line with the 11-point right-hand side of FILTRONPER
+ * (filtro_incomp.f90:307-362), xi and eta with the non-periodic arrays,
+ * zeta with the periodic ones (as the reference passes them), then times
+ * 1/J (jbn 11); um1 in place, then u, temp and pres recomputed (u =
+ * um1(1+c)/um1(1), temp = um1(5), pres = um1(1)/(Mach*Mach)).
+ * 0 <= cfilt < 0.5, mach > 0 (EMX_ERR_ARG otherwise).  The first call per
+ * (line length, cfilt) builds the line tables (device allocation). */
+int emx_filtrado(emx_ctx* ctx, int itf, double* um1, const double* jbn, double cfilt,
+                 double mach, double* u, double* pres, double* temp);
+
 /* Pressure extrapolation into the solid (sobject > 0): advect_CUDA
  * (src/LevelSet/lsTransportEc.cu:24-197 -> interface.cu:341-436).
  * dx,dy,dz are the computational spacings the reference passes

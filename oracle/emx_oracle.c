@@ -1385,3 +1385,103 @@ int orc_rk_rebuild(long n, double *um1, const double *u, const double *pres, dou
     }
     return 0;
 }
+
+
+/* ---- compact filter ------------------------------------------------------ */
+
+int orc_filter_coeffs(int periodic, int n, double cfilt, double *a, double *b, double *c)
+{
+    if (n < 2) return -3;
+    for (int l = 0; l < n; l++) {
+        b[l] = 1.0;
+        /* INIFILTRO sets cfilt2..cfilt5 to zero on both branches */
+        const int interior = l + 1 >= 6 && l + 1 <= n - 5;
+        a[l] = c[l] = (periodic || interior) ? cfilt : 0.0;
+    }
+    return 0;
+}
+
+/* filtronper: 11-point explicit RHS on rows 6..n-5 (1-based), identity on
+ * the five rows at each end, then the Thomas algorithm in the reference's
+ * form (division by the pivot). */
+int orc_filtronper(int n, double *u, const double *a, const double *b, const double *c,
+                   double cfilt)
+{
+    if (n < 2) return -3;
+    const double arsf = (193. + 126. * cfilt) / 256.;
+    const double brsf = (105. + 302. * cfilt) / 256.;
+    const double crsf = (-15. + 30. * cfilt) / 64.;
+    const double drsf = (45. - 90. * cfilt) / 512.;
+    const double ersf = (-5. + 10. * cfilt) / 256.;
+    const double frsf = (1. - 2 * cfilt) / 512.;
+    double *r = malloc(sizeof(double) * (size_t)n);
+    double *gam = malloc(sizeof(double) * (size_t)(n + 1));
+    if (!r || !gam) { free(r); free(gam); return -5; }
+    for (int l = 0; l < n; l++) {
+        if (l + 1 >= 6 && l + 1 <= n - 5)
+            r[l] = 0.5 * (arsf * (u[l] + u[l]) + brsf * (u[l + 1] + u[l - 1]) +
+                          crsf * (u[l + 2] + u[l - 2]) + drsf * (u[l + 3] + u[l - 3]) +
+                          ersf * (u[l + 4] + u[l - 4]) + frsf * (u[l + 5] + u[l - 5]));
+        else
+            r[l] = u[l];
+    }
+    double bet = b[0];
+    u[0] = r[0] / bet;
+    for (int l = 1; l < n; l++) {
+        gam[l] = c[l - 1] / bet;
+        bet = b[l] - a[l] * gam[l];
+        u[l] = (r[l] - a[l] * u[l - 1]) / bet;
+    }
+    for (int l = n - 2; l >= 0; l--) u[l] = u[l] - gam[l + 1] * u[l + 1];
+    free(r);
+    free(gam);
+    return 0;
+}
+
+/* one directional pass of FILTRADO over field m */
+static int filter_dir(int dir, int nx, int ny, int nz, double *f, const double *j10,
+                      const double *j11, double cfilt)
+{
+    const int n = dir == 0 ? nx : (dir == 1 ? ny : nz);
+    const long sl = dir == 0 ? 1 : (dir == 1 ? nx : (long)nx * ny);
+    double *a = malloc(sizeof(double) * (size_t)n), *b = malloc(sizeof(double) * (size_t)n);
+    double *c = malloc(sizeof(double) * (size_t)n), *line = malloc(sizeof(double) * (size_t)n);
+    if (!a || !b || !c || !line) { free(a); free(b); free(c); free(line); return -5; }
+    orc_filter_coeffs(dir == 2, n, cfilt, a, b, c);
+    const int n1 = dir == 0 ? ny : nx, n2 = dir == 2 ? ny : nz;
+    int rc = 0;
+    for (int o2 = 0; o2 < n2 && !rc; o2++)
+        for (int o1 = 0; o1 < n1 && !rc; o1++) {
+            long base;
+            if (dir == 0) base = (long)nx * (o1 + (long)ny * o2);
+            else if (dir == 1) base = o1 + (long)nx * ny * o2;
+            else base = o1 + (long)nx * o2;
+            for (int l = 0; l < n; l++) line[l] = f[base + l * sl] * j10[base + l * sl];
+            rc = orc_filtronper(n, line, a, b, c, cfilt);
+            for (int l = 0; l < n; l++) f[base + l * sl] = line[l] * j11[base + l * sl];
+        }
+    free(a); free(b); free(c); free(line);
+    return rc;
+}
+
+int orc_filtrado(int itf, int nx, int ny, int nz, double *um1, const double *jbn, double cfilt,
+                 double mach, double *u, double *pres, double *temp)
+{
+    /* direction order of each itf (herramientas_incomp2.f90:245-392) */
+    static const int order[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2},
+                                    {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    if (itf < 1 || itf > 6) return -1;
+    const long N = (long)nx * ny * nz;
+    for (int m = 0; m < 5; m++)
+        for (int k = 0; k < 3; k++) {
+            int rc = filter_dir(order[itf - 1][k], nx, ny, nz, um1 + m * N, jbn + 9 * N,
+                                jbn + 10 * N, cfilt);
+            if (rc) return rc;
+        }
+    for (long p = 0; p < N; p++) {
+        for (int c = 0; c < 3; c++) u[c * N + p] = um1[(1 + c) * N + p] / um1[p];
+        temp[p] = um1[4 * N + p];
+        pres[p] = um1[p] / (mach * mach);
+    }
+    return 0;
+}

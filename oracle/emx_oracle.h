@@ -152,6 +152,22 @@ int orc_rk_clip(long n, double *um1, const double *mask, double mach, double *u,
  * um1(1) = pres*Mach*Mach, um1(1+c) = u(c)*um1(1). */
 int orc_rk_rebuild(long n, double *um1, const double *u, const double *pres, double mach);
 
+/* Compact filter.  orc_filter_coeffs: the tridiagonal a, b, c of one line
+ * (INIFILTRO, src/MSVSingle/filtro_incomp.f90:2-139): periodic = 0 gives the
+ * non-periodic arrays (a = c = cfilt on 1-based rows 6..n-5, zero on the
+ * five rows at each end), periodic = 1 the periodic ones (a = c = cfilt on
+ * every row).  orc_filtronper: one line in place (filtro_incomp.f90:307-362,
+ * RHS constants :140-145).  orc_filtrado: FILTRADO(itf)
+ * (herramientas_incomp2.f90:227-410) on um1 (nx,ny,nz,5) in place -- each
+ * field times J, filtered along the three directions in the order itf
+ * selects (xi and eta with the non-periodic arrays, zeta with the periodic
+ * ones), times 1/J -- then u, temp, pres recomputed. */
+int orc_filter_coeffs(int periodic, int n, double cfilt, double *a, double *b, double *c);
+int orc_filtronper(int n, double *u, const double *a, const double *b, const double *c,
+                   double cfilt);
+int orc_filtrado(int itf, int nx, int ny, int nz, double *um1, const double *jbn, double cfilt,
+                 double mach, double *u, double *pres, double *temp);
+
 /* ---- learned closure network (no reference counterpart) ------------------ */
 /* 10 -> 64 -> 64 -> 64 -> 1, SiLU hidden activations, softplus output.
  * params (9089): W1[64][10] b1[64] W2[64][64] b2[64] W3[64][64] b3[64]

@@ -63,6 +63,9 @@ def lib():
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
             "orc_rk_update": [_I, _L, _P, _P, _P, _P, _P, _D],
             "orc_frontera_val": [_I, _I, _I, _P],
+            "orc_filter_coeffs": [_I, _I, _D, _P, _P, _P],
+            "orc_filtronper": [_I, _P, _P, _P, _P, _D],
+            "orc_filtrado": [_I, _I, _I, _I, _P, _P, _D, _D, _P, _P, _P],
             "orc_varc_to_varnc": [_L, _P, _D, _P, _P, _P, _P],
             "orc_varnc_to_varc": [_L, _P, _P, _P, _D, _P, _P],
             "orc_rk_clip": [_L, _P, _P, _D, _P, _P, _P, _P],
@@ -233,6 +236,32 @@ def rk_rebuild(um1, u, pres, mach):
     _check(lib().orc_rk_rebuild(n, _ptr(um1), _ptr(np.ascontiguousarray(u)),
                                 _ptr(np.ascontiguousarray(pres)), mach), "rk_rebuild")
     return um1
+
+
+def filter_coeffs(periodic, n, cfilt):
+    a, b, c = np.empty(n), np.empty(n), np.empty(n)
+    _check(lib().orc_filter_coeffs(int(periodic), n, cfilt, _ptr(a), _ptr(b), _ptr(c)),
+           "filter_coeffs")
+    return a, b, c
+
+
+def filtronper(line, periodic, cfilt):
+    """One line through filtronper with INIFILTRO's arrays."""
+    u = np.array(line, dtype=np.float64, copy=True)
+    a, b, c = filter_coeffs(periodic, u.size, cfilt)
+    _check(lib().orc_filtronper(u.size, _ptr(u), _ptr(a), _ptr(b), _ptr(c), cfilt), "filtronper")
+    return u
+
+
+def filtrado(itf, um1, jbn, cfilt, mach):
+    """FILTRADO(itf) on a copy of um1; returns um1, u, pres, temp."""
+    out = np.array(um1, dtype=np.float64, copy=True)
+    nz, ny, nx = out.shape[1:]
+    u = np.empty((3,) + out.shape[1:])
+    pres, temp = np.empty(out.shape[1:]), np.empty(out.shape[1:])
+    _check(lib().orc_filtrado(itf, nx, ny, nz, _ptr(out), _ptr(np.ascontiguousarray(jbn)), cfilt,
+                              mach, _ptr(u), _ptr(pres), _ptr(temp)), "filtrado")
+    return out, u, pres, temp
 
 
 def _f32(a):
@@ -427,6 +456,17 @@ class RefFortran:
                                  _ptr(np.ascontiguousarray(temp)), ctypes.c_double(mach),
                                  _ptr(um0), _ptr(pres0))
         return um0, pres0
+
+    def filtrado(self, itf, um1, jbn, cfilt, mach):
+        """INIFILTRO(cfilt) then FILTRADO(itf); returns um1, u, pres, temp."""
+        self.init(*dims(um1))
+        out = np.array(um1, dtype=np.float64, copy=True)
+        u = np.empty((3,) + um1.shape[1:])
+        temp, pres = np.empty(um1.shape[1:]), np.empty(um1.shape[1:])
+        self.L.ref_filtrado(_I(itf), _ptr(out), _ptr(np.ascontiguousarray(jbn)),
+                            ctypes.c_double(cfilt), ctypes.c_double(mach), _ptr(u), _ptr(temp),
+                            _ptr(pres))
+        return out, u, pres, temp
 
     def jacobeano(self, x):
         self.init(*dims(x))

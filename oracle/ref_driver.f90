@@ -429,3 +429,49 @@ subroutine ref_varnc_to_varc(uin, pin, tin, machin, umout, p0out) &
   p0out = pres0
   deallocate(um0)
 end subroutine ref_varnc_to_varc
+
+! ---- compact filter ---------------------------------------------------------
+
+! INIFILTRO with the given cfilt (src/MSVSingle/filtro_incomp.f90:2-166),
+! then FILTRADO(itf) on um1 (herramientas_incomp2.f90:227-410); returns the
+! filtered um1 and the u, temp, pres it recomputes.
+subroutine ref_filtrado(itf, umio, jin, cfiltin, machin, uout, tout, pout) &
+    bind(C, name='ref_filtrado')
+  use iso_c_binding
+  use dimensiones
+  use variables
+  use velocidades
+  use jacobtools
+  use consfiltro
+  use consderper_f
+  use consdernper_f
+  use flow
+  implicit none
+  integer(c_int), value :: itf
+  real(c_double), intent(inout) :: umio(nx,ny,nz,5)
+  real(c_double), intent(in) :: jin(nx,ny,nz,11)
+  real(c_double), value :: cfiltin, machin
+  real(c_double), intent(out) :: uout(nx,ny,nz,3), tout(nx,ny,nz), pout(nx,ny,nz)
+  integer :: nmcit
+  if (allocated(um1)) deallocate(um1)
+  allocate(um1(nx,ny,nz,5))
+  if (allocated(axnpf)) deallocate(axnpf, aynpf, aznpf, bxnpf, bynpf, bznpf, &
+                                   cxnpf, cynpf, cznpf)
+  if (allocated(axpf)) deallocate(axpf, aypf, azpf, bxpf, bypf, bzpf, cxpf, cypf, czpf)
+  allocate(axnpf(nx), bxnpf(nx), cxnpf(nx), aynpf(ny), bynpf(ny), cynpf(ny))
+  allocate(aznpf(nz), bznpf(nz), cznpf(nz))
+  allocate(axpf(nx), bxpf(nx), cxpf(nx), aypf(ny), bypf(ny), cypf(ny))
+  allocate(azpf(nz), bzpf(nz), czpf(nz))
+  cfilt = cfiltin
+  mach = machin
+  call inifiltro()
+  um1 = umio
+  jbn = jin
+  nmcit = 1
+  call filtrado(itf, nmcit)
+  umio = um1
+  uout = u
+  tout = temp
+  pout = pres
+  deallocate(um1)
+end subroutine ref_filtrado

@@ -7,6 +7,7 @@ container only (the reference tree is not shipped):
 
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py glue     # the RUNGEK glue fixture only
+    python tests/golden/make_golden.py filter   # the compact filter fixture only
 """
 import os
 import sys
@@ -89,12 +90,57 @@ def make_glue():
     return out
 
 
+FILTER_CASES = [
+    # name, (nx, ny, nz), itf values, cfilt, stored fields of um1
+    ("g12x11x11", (12, 11, 11), (1, 2, 3, 4, 5, 6), 0.49, (0, 2)),
+    ("g21x12x13", (21, 12, 13), (1,), 0.45, (1, 4)),
+    ("g9x8x7", (9, 8, 7), (1, 4), 0.49, (0, 1, 2, 3, 4)),
+]
+
+
+def make_filter():
+    """FILTRADO(itf) of the reference on a seeded conservative state over a
+    stretched grid, for every direction order, two cfilt values and a grid
+    whose lines are all boundary rows.  The five fields are filtered
+    independently, so a case stores the fields listed (and, for its first
+    itf, the u, pres, temp FILTRADO recomputes); jbn is stored as J, 1/J,
+    the only components the filter reads."""
+    ref = RefFortran("c6f")
+    out = {}
+    for name, shape, itfs, cfilt, keep in FILTER_CASES:
+        nx, ny, nz = shape
+        ref.init(nx, ny, nz)
+        flds = fields.stage_fields(nx, ny, nz, 20261212, curvilinear=True, turbulent=False)
+        jbn = ref.jacobeano(np64(flds["x"]))
+        jbn[:9] = 0.0
+        rng = np.random.default_rng(sum(shape))
+        um1 = rng.standard_normal((5, nz, ny, nx))
+        um1[0] = 1.0 + 0.2 * rng.random((nz, ny, nx))
+        out[f"{name}_um1"] = um1
+        out[f"{name}_jac"] = jbn[9:11]
+        out[f"{name}_cfilt"] = np.array(cfilt)
+        out[f"{name}_keep"] = np.array(keep)
+        for itf in itfs:
+            f, u, pres, temp = ref.filtrado(itf, um1, jbn, cfilt, 0.3)
+            out[f"{name}_itf{itf}_um1"] = f[list(keep)]
+            if itf == itfs[0]:
+                out[f"{name}_itf{itf}_u"] = u
+                out[f"{name}_itf{itf}_pres"] = pres
+                out[f"{name}_itf{itf}_temp"] = temp
+    return out
+
+
 def main():
     if sys.argv[1:] == ["glue"]:
         np.savez_compressed(os.path.join(HERE, "ref_glue_g9x8x7.npz"), **make_glue())
         print("glue vectors written to", HERE)
         return
+    if sys.argv[1:] == ["filter"]:
+        np.savez_compressed(os.path.join(HERE, "ref_filter.npz"), **make_filter())
+        print("filter vectors written to", HERE)
+        return
     np.savez_compressed(os.path.join(HERE, "ref_glue_g9x8x7.npz"), **make_glue())
+    np.savez_compressed(os.path.join(HERE, "ref_filter.npz"), **make_filter())
     for scheme in ("c4", "c6f"):
         if not RefFortran.available(scheme):
             raise SystemExit("oracle/_ref is not built: run `make -C oracle ref`")

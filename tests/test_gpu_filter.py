@@ -1,0 +1,87 @@
+"""Compact filter FILTRADO on the GPU (line-solver kernels with the filter's
+matrix and 11-point right-hand side) against the oracle, which
+test_oracle_filter.py pins to the reference Fortran bit for bit.
+
+Tolerance 1e-12 normwise relative per field (the chunked solve with
+precomputed reciprocal pivots differs from the Fortran's divisions by
+rounding only); linearity at the 256^3 size."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import normwise_rel
+from emix3d_amd import Operator, fields
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def case(shape, seed, oracle):
+    nx, ny, nz = shape
+    f = fields.stage_fields(nx, ny, nz, seed, curvilinear=True, turbulent=False)
+    jbn = oracle.jacobeano(2, np.ascontiguousarray(f["x"].numpy()))
+    rng = np.random.default_rng(seed)
+    um1 = rng.standard_normal((5, nz, ny, nx))
+    um1[0] = 1.0 + 0.2 * rng.random((nz, ny, nx))
+    return um1, jbn
+
+
+@pytest.mark.parametrize("shape,itfs", [((12, 11, 11), (1, 2, 3, 4, 5, 6)), ((9, 8, 7), (1, 4)),
+                                        ((70, 33, 41), (1, 6)), ((256, 20, 24), (1,)),
+                                        ((40, 300, 17), (3,)), ((30, 20, 512), (5,))])
+@pytest.mark.parametrize("cfilt", [0.49, 0.3])
+def test_filtrado_vs_oracle(oracle, shape, itfs, cfilt):
+    um1, jbn = case(shape, sum(shape), oracle)
+    op = Operator(*shape, "c6c")
+    for itf in itfs:
+        print(f"  filter {shape} itf {itf} cfilt {cfilt}", flush=True)
+        ref, ru, rp, rt = oracle.filtrado(itf, um1, jbn, cfilt, 0.3)
+        d = dev(um1)
+        u, p, t = op.filtrado(itf, d, dev(jbn), cfilt, 0.3)
+        got = host(d)
+        for m in range(5):
+            err = normwise_rel(got[m], ref[m])
+            assert err <= TOL, (itf, m, err)
+        assert normwise_rel(host(u), ru) <= TOL
+        assert normwise_rel(host(p), rp) <= TOL
+        assert np.array_equal(host(t), got[4])
+
+
+def test_filtrado_linear_full_size():
+    """At 256^3: the filter is linear in um1 for fixed J (each field is
+    filtered on its own), checked on two random states and their sum."""
+    n = 256
+    op = Operator(n, n, n, "c6f")
+    x = fields.grid_tanh(n, n, n, device=torch.device("cuda"))
+    jbn = op.jacobeano(x)
+    gen = torch.Generator(device="cuda").manual_seed(12)
+    a = torch.randn((5, n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    b = torch.randn((5, n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    a[0] += 4.0
+    b[0] += 4.0
+    s = a + b
+    for t in (a, b, s):
+        op.filtrado(1, t, jbn, 0.49, 0.3)
+    err = float(torch.linalg.vector_norm(s - a - b) / torch.linalg.vector_norm(s))
+    print(f"  linearity 256^3: {err:.2e}", flush=True)
+    assert err < 1e-14
+
+
+def test_filtrado_argument_checks():
+    from emix3d_amd._lib import EmxError
+    op = Operator(8, 8, 8, "c6c")
+    um1 = torch.ones((5, 8, 8, 8), dtype=torch.float64, device="cuda")
+    jbn = torch.ones((11, 8, 8, 8), dtype=torch.float64, device="cuda")
+    for itf, cf, mach in ((0, 0.4, 0.3), (7, 0.4, 0.3), (1, 0.5, 0.3), (1, -0.1, 0.3),
+                          (1, 0.4, 0.0)):
+        with pytest.raises(EmxError):
+            op.filtrado(itf, um1, jbn, cf, mach)
