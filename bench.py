@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-glue", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer 5 times (the reference kernel for "
                          "scaling hardware byte counters) and exit")
@@ -138,6 +139,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_section(args.steps, args.warmup)
+    glue = None
+    if not args.no_glue:
+        glue = glue_section(op, f, jbn, sgd, c4, work, dt, args.steps, args.warmup)
 
     cpu = None
     parity = None
@@ -170,6 +174,7 @@ def main():
         "parity": parity,
         "batched": batched,
         "learned": learned,
+        "substep_and_filter": glue,
     }
     print(json.dumps(out), flush=True)
 
@@ -227,6 +232,61 @@ def timed(fn, steps, warmup):
 def rel_l2(a, b):
     import torch
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def glue_section(op, f, jbn, sgd, c4, work, stage_s, steps, warmup):
+    """One whole device RUNGEK sub-step (operator + SSP-RK3 update +
+    boundary values + velocity clip + pressure extrapolation into a solid +
+    rebuild) and the compact filter FILTRADO, at the benchmark size."""
+    import torch
+    from emix3d_amd import fields
+    n = f["u"].shape[-1]
+    N = float(n) ** 3
+    mach, dt = 0.3, 1e-3
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    st = {k: f[k].clone() for k in ("um", "u", "temp", "pres")}
+    um0 = f["um"].clone()
+    mask = (torch.rand(f["pres"].shape, dtype=torch.float64, device="cuda",
+                       generator=gen) > 0.02).double()
+    x = fields.grid_uniform(n, n, n, device=torch.device("cuda"))
+    sobject = (-fields.level_set_sphere_surface(x, 5)).contiguous()
+    dxyz = (1.0 / (n - 1),) * 3
+    um1 = op._empty(5)
+
+    def substep():
+        op.rk_substep(2, um0, st["um"], st["u"], st["temp"], st["pres"], jbn, sgd, c4, dt, mach,
+                      mask, sobject=sobject, dxyz=dxyz, deltamin=dxyz[0], work=work, um1=um1)
+
+    t_sub = timed(substep, steps, warmup)
+    op.timing_begin()
+    substep()
+    torch.cuda.synchronize()
+    glue_k = {}
+    for k in op.timing_end():
+        if k["label"] in ("rk_update", "frontera_z", "frontera_y", "frontera_x", "rk_clip",
+                          "rk_rebuild") or k["label"].startswith("ls_"):
+            key = k["label"] + (f" dir{k['dir']}" if k["dir"] >= 0 else "")
+            glue_k[key] = {"launches": k["launches"], "us": k["ms"] * 1e3,
+                           "GBs": k["bytes"] / (k["ms"] * 1e-3) / 1e9 if k["ms"] > 0 else 0.0}
+    fu = {"um1": f["um"].clone()}
+    u3, p1, t1 = op._empty(3), op._empty(), op._empty()
+
+    def filt():
+        op.filtrado(1, fu["um1"], jbn, 0.49, mach, u=u3, pres=p1, temp=t1)
+
+    t_f = timed(filt, max(3, steps // 2), 1)
+    # FILTRADO: 15 line passes (field, J, 1/J in; field out) + the
+    # primitive variables (um1 1..5 in; u, pres, temp out)
+    f_bytes = (15 * 4 + 10) * 8.0 * N
+    out = {"workload": f"{n}^3 device RUNGEK sub-step 2 (operator, update, boundary values, "
+                       "clip, pressure extrapolation into a sphere, rebuild) and FILTRADO(itf=1, "
+                       "cfilt=0.49)",
+           "substep_ms": t_sub * 1e3, "stage_ms": stage_s * 1e3,
+           "substep_pts_per_s": N / t_sub, "glue_kernels": glue_k,
+           "filter_ms": t_f * 1e3, "filter_GBs_algorithmic": f_bytes / t_f / 1e9,
+           "filter_frac_of_hbm_peak": f_bytes / t_f / 1e9 / HBM_PEAK_GBS}
+    log(f"bench: sub-step and filter {out}")
+    return out
 
 
 def learned_section(op, f, jbn, sgd, c4, work, steps, warmup):
