@@ -53,14 +53,12 @@ int op_deriv(emx_ctx* ctx, int dir, const double* f, double* df, double ih) {
 // Derivatives of m independent fields along dir, two fields per launch (a
 // line solve holds 2*(2C+6) doubles per field in registers; pairs keep the
 // strided kernel within the 128 VGPRs its 1024-thread workgroups allow).
+// Pairs keep twice the bytes in flight per work-item: measured faster than
+// one field per launch on every kind with a working set beyond the L3.
 static int deriv_fields(emx_ctx* ctx, int dir, int m, const double* const* in,
                         double* const* out, double ih) {
   int f = 0;
-  // pairs of fields per launch on the wavefront kind; one field per launch
-  // on the strided kinds, whose single-field form moves more bytes per
-  // second (fewer registers, more workgroups in flight)
-  const bool pairs = ctx->plan[PLAN_DERIV][dir].kind == KIND_WAVE;
-  for (; pairs && f + 1 < m; f += 2) {
+  for (; f + 1 < m; f += 2) {
     ProbDeriv<2> p;
     p.in[0] = in[f];
     p.in[1] = in[f + 1];

@@ -169,10 +169,12 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
   int pass = 0, k = 0;
   for (int dir = 0; dir < 3; dir++) {
     if (!(mat & (1 << dir))) continue;
-    for (int slot = 0; slot < 8; slot++) {
-      double* out = slot < 5 ? rs + slot * N : rsv + (slot - 4) * N;
-      ProbAccum p{ctx->fluxbuf + (8 * k + slot) * N, ctx->acc + slot * N, out, fp.jbn + 10 * N,
-                  pass};
+    // slots (0,1) (2,3) (4,5) (6,7): outputs rs 1..5 then rsv 2..4
+    for (int slot = 0; slot < 8; slot += 2) {
+      double* o0 = slot < 5 ? rs + slot * N : rsv + (slot - 4) * N;
+      double* o1 = slot + 1 < 5 ? rs + (slot + 1) * N : rsv + (slot - 3) * N;
+      ProbAccum p{ctx->fluxbuf + (8 * k + slot) * N, ctx->acc + slot * N, o0, o1,
+                  fp.jbn + 10 * N, N, pass};
       EMX_TRY(launch_lines(ctx, dir, p, ih[dir]));
     }
     k++;
