@@ -248,8 +248,10 @@ def glue_section(op, f, jbn, sgd, c4, work, stage_s, steps, warmup):
     um0 = f["um"].clone()
     mask = (torch.rand(f["pres"].shape, dtype=torch.float64, device="cuda",
                        generator=gen) > 0.02).double()
+    # a solid ball of radius 0.15 at the centre of the unit box (sobject > 0
+    # inside, the signed distance outside): about 1.4% of the points
     x = fields.grid_uniform(n, n, n, device=torch.device("cuda"))
-    sobject = (-fields.level_set_sphere_surface(x, 5)).contiguous()
+    sobject = (0.15 - torch.sqrt(((x - 0.5) ** 2).sum(0))).contiguous()
     dxyz = (1.0 / (n - 1),) * 3
     um1 = op._empty(5)
 
@@ -279,8 +281,8 @@ def glue_section(op, f, jbn, sgd, c4, work, stage_s, steps, warmup):
     # primitive variables (um1 1..5 in; u, pres, temp out)
     f_bytes = (15 * 4 + 10) * 8.0 * N
     out = {"workload": f"{n}^3 device RUNGEK sub-step 2 (operator, update, boundary values, "
-                       "clip, pressure extrapolation into a sphere, rebuild) and FILTRADO(itf=1, "
-                       "cfilt=0.49)",
+                       "clip, pressure extrapolation into a central solid ball of radius 0.15, "
+                       "rebuild) and FILTRADO(itf=1, cfilt=0.49)",
            "substep_ms": t_sub * 1e3, "stage_ms": stage_s * 1e3,
            "substep_pts_per_s": N / t_sub, "glue_kernels": glue_k,
            "filter_ms": t_f * 1e3, "filter_GBs_algorithmic": f_bytes / t_f / 1e9,

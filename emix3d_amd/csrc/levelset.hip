@@ -89,55 +89,137 @@ __global__ void k_ls_grad1(const double* S, const double* j1, const double* j5,
   }
 }
 
-// extrapolKernel (extrapol.cu:5-103, extFlag>0) fused with the SSP-RK3
-// stage update RunGK_{First,Second,Third}S (extrapol.cu:155-212):
+// extrapolKernel (extrapol.cu:5-103, extFlag>0) at one solid point (phiS >
+// 0) of the padded grid: the upwind metric-scaled one-sided differences of
+// ext along the sign of the solid's gradient.  Elsewhere the reference's rs
+// is exactly zero (so = 0 and finite operands).
+__device__ __forceinline__ double ls_extrap_rs(long long id, const double* ext, const double* j1,
+                                               const double* j5, const double* j9,
+                                               const double* dphi, PDims g, double iX,
+                                               double iY, double iZ) {
+  const long long NG = g.N();
+  const long long sxy = (long long)g.x * g.y;
+  const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
+  const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
+            z = (int)(row / (unsigned)g.y);
+  long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x, kp = id + sxy,
+            km = id - sxy;
+  long long id2 = id;
+  if (x == 0) { id2 = ip; im = id; }
+  if (y == 0) { id2 = jp; jm = id; }
+  if (z == 0) { id2 = kp; km = id; }
+  if (x == g.x - 1) { id2 = im; ip = id; }
+  if (y == g.y - 1) { id2 = jm; jp = id; }
+  if (z == g.z - 1) { id2 = km; kp = id; }
+  const double so = 1.0;
+  const double pdx = so * dphi[id], pdy = so * dphi[NG + id], pdz = so * dphi[2 * NG + id];
+  const double e2 = ext[id2];
+  const double xd = iX * j1[id] * (e2 - ext[im]);
+  const double yd = iY * j5[id] * (e2 - ext[jm]);
+  const double zd = iZ * j9[id] * (e2 - ext[km]);
+  const double xu = iX * j1[id] * (ext[ip] - e2);
+  const double yu = iY * j5[id] * (ext[jp] - e2);
+  const double zu = iZ * j9[id] * (ext[kp] - e2);
+  double s = (double)(pdx > 0.0) - (double)(pdx < 0.0);
+  double r = (0.5 * (s + 1.0) * xd + 0.5 * fabs(s - 1.0) * xu) * pdx;
+  s = (double)(pdy > 0.0) - (double)(pdy < 0.0);
+  r += (0.5 * (s + 1.0) * yd + 0.5 * fabs(s - 1.0) * yu) * pdy;
+  s = (double)(pdz > 0.0) - (double)(pdz < 0.0);
+  r += (0.5 * (s + 1.0) * zd + 0.5 * fabs(s - 1.0) * zu) * pdz;
+  return r;
+}
+
+// Ordered list of the solid points (S > 0) of the padded grid: per-chunk
+// counts, one exclusive scan, then each chunk writes its indices in order
+// (wavefront ballots, no atomics).
+constexpr int LS_CHUNK = 4096;
+
+__global__ void k_ls_solid_count(const double* S, long long NG, int* bcount) {
+  __shared__ int wsum[4];
+  const long long base = (long long)blockIdx.x * LS_CHUNK;
+  int c = 0;
+  for (int t = threadIdx.x; t < LS_CHUNK; t += 256) {
+    const long long id = base + t;
+    c += (id < NG && S[id] > 0.0) ? 1 : 0;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) bcount[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// Exclusive scan of nb chunk counts in place (one workgroup of 1024); the
+// total into *total.
+__global__ void __launch_bounds__(1024) k_ls_scan(int* bcount, int nb, int* total) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = t * per < nb ? t * per : nb;
+  const int b1 = b0 + per < nb ? b0 + per : nb;
+  int s = 0;
+  for (int b = b0; b < b1; b++) s += bcount[b];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? part[t - 1] : 0;
+  for (int b = b0; b < b1; b++) {
+    const int c = bcount[b];
+    bcount[b] = run;
+    run += c;
+  }
+  if (t == 1023) *total = part[1023];
+}
+
+__global__ void k_ls_solid_write(const double* S, long long NG, const int* boff, int* list) {
+  __shared__ int wsum[4];
+  const long long base = (long long)blockIdx.x * LS_CHUNK;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int run = boff[blockIdx.x];
+  for (int r = 0; r < LS_CHUNK / 256; r++) {
+    const long long id = base + r * 256 + threadIdx.x;
+    const bool f = id < NG && S[id] > 0.0;
+    const unsigned long long m = __ballot(f);
+    const int pre = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wv] = __popcll(m);
+    __syncthreads();
+    int off = run;
+    for (int w = 0; w < wv; w++) off += wsum[w];
+    if (f) list[off + pre] = (int)id;
+    run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+}
+
+__global__ void k_copy2(const double* a, double* b, double* c, long long n) {
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long long)gridDim.x * blockDim.x) {
+    const double v = a[p];
+    b[p] = v;
+    c[p] = v;
+  }
+}
+
+// One SSP-RK3 stage of the extrapolation (RunGK_{First,Second,Third}S,
+// extrapol.cu:155-212) at the listed solid points:
 //   stage 0: out = ext - dt*rs
 //   stage 1: out = 0.75*P + 0.25*(ext - dt*rs)
 //   stage 2: out = (P + 2*(ext - dt*rs))/3      (out may alias P)
-// rs is exactly zero where phiS <= 0 (so = 0 and finite operands), so the
-// stencil is skipped there.
-__global__ void k_ls_extrap_stage(int stage, const double* ext,
-                                  const double* P, const double* S,
-                                  const double* j1, const double* j5,
-                                  const double* j9, const double* dphi,
-                                  PDims g, double iX, double iY, double iZ,
-                                  double dt, double* out) {
-  const long long NG = g.N();
-  const long long sxy = (long long)g.x * g.y;
-  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
-       id += (long long)gridDim.x * blockDim.x) {
-    double rs = 0.0;
-    if (S[id] > 0.0) {
-      const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
-      const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
-                z = (int)(row / (unsigned)g.y);
-      long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x,
-                kp = id + sxy, km = id - sxy;
-      long long id2 = id;
-      if (x == 0) { id2 = ip; im = id; }
-      if (y == 0) { id2 = jp; jm = id; }
-      if (z == 0) { id2 = kp; km = id; }
-      if (x == g.x - 1) { id2 = im; ip = id; }
-      if (y == g.y - 1) { id2 = jm; jp = id; }
-      if (z == g.z - 1) { id2 = km; kp = id; }
-      const double so = 1.0;
-      const double pdx = so * dphi[id], pdy = so * dphi[NG + id],
-                   pdz = so * dphi[2 * NG + id];
-      const double e2 = ext[id2];
-      const double xd = iX * j1[id] * (e2 - ext[im]);
-      const double yd = iY * j5[id] * (e2 - ext[jm]);
-      const double zd = iZ * j9[id] * (e2 - ext[km]);
-      const double xu = iX * j1[id] * (ext[ip] - e2);
-      const double yu = iY * j5[id] * (ext[jp] - e2);
-      const double zu = iZ * j9[id] * (ext[kp] - e2);
-      double s = (double)(pdx > 0.0) - (double)(pdx < 0.0);
-      double r = (0.5 * (s + 1.0) * xd + 0.5 * fabs(s - 1.0) * xu) * pdx;
-      s = (double)(pdy > 0.0) - (double)(pdy < 0.0);
-      r += (0.5 * (s + 1.0) * yd + 0.5 * fabs(s - 1.0) * yu) * pdy;
-      s = (double)(pdz > 0.0) - (double)(pdz < 0.0);
-      r += (0.5 * (s + 1.0) * zd + 0.5 * fabs(s - 1.0) * zu) * pdz;
-      rs = r;
-    }
+// The fluid points keep P in all three buffers: their rs is zero, and the
+// stages only re-round them (0.75*P + 0.25*P, (P + 2*P)/3), which the
+// reference does and this build does not.
+__global__ void k_ls_extrap_list(int stage, const int* list, const int* count, const double* ext,
+                                 const double* P, const double* j1, const double* j5,
+                                 const double* j9, const double* dphi, PDims g, double iX,
+                                 double iY, double iZ, double dt, double* out) {
+  const int n = *count;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const long long id = list[t];
+    const double rs = ls_extrap_rs(id, ext, j1, j5, j9, dphi, g, iX, iY, iZ);
     const double ev = ext[id];
     if (stage == 0) out[id] = ev - dt * rs;
     else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (ev - dt * rs);
@@ -375,7 +457,13 @@ PDims padded(const Dims& d) { return PDims{d.nx + 2 * GC, d.ny + 2 * GC, d.nz + 
 
 }  // namespace
 
-size_t ls_scratch_doubles(const Dims& d) { return (size_t)10 * padded(d).N(); }
+// ten padded fields, then the solid-point list of the extrapolation: chunk
+// counts, the count and up to one index per padded point (ints)
+size_t ls_scratch_doubles(const Dims& d) {
+  const size_t NG = (size_t)padded(d).N();
+  const size_t ints = (NG + LS_CHUNK - 1) / LS_CHUNK + 1 + NG;
+  return 10 * NG + (ints + 1) / 2 + 1;
+}
 
 long long ls_padded_points(int nx, int ny, int nz) {
   return (long long)(nx + 2 * GC) * (ny + 2 * GC) * (nz + 2 * GC);
@@ -432,16 +520,42 @@ int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
   EMX_TIMED(ctx, "ls_grad1", 56.0 * NG,
           hipLaunchKernelGGL(k_ls_grad1, dim3(bg), dim3(256), 0, st, S, J1, J5, J9, g,
                      iX, iY, iZ, dphi));
+  // the solid points, in order; D1 = D2 = P at every point (the fluid
+  // points are never written again)
+  const int nb = (int)((NG + LS_CHUNK - 1) / LS_CHUNK);
+  int* bcount = reinterpret_cast<int*>(D2 + NG);
+  int* count = bcount + nb;
+  int* list = count + 1;
+  EMX_TIMED(ctx, "ls_solid_list", 8.0 * NG,
+            hipLaunchKernelGGL(k_ls_solid_count, dim3(nb), dim3(256), 0, st, S, NG, bcount));
+  EMX_TIMED(ctx, "ls_solid_list", 0.0,
+            hipLaunchKernelGGL(k_ls_scan, dim3(1), dim3(1024), 0, st, bcount, nb, count));
+  EMX_TIMED(ctx, "ls_solid_list", 8.0 * NG,
+            hipLaunchKernelGGL(k_ls_solid_write, dim3(nb), dim3(256), 0, st, S, NG, bcount, list));
+  EMX_TIMED(ctx, "ls_copy", 24.0 * NG,
+            hipLaunchKernelGGL(k_copy2, dim3(bg), dim3(256), 0, st, P, D1, D2, NG));
+  EMX_TRY_HIP(hipGetLastError());
+  // algorithmic bytes of a stage per solid point (only known on the device;
+  // read back while the context records timings): list entry, ext, three
+  // metrics, three gradient components, P (stages 1, 2), out
+  double nsolid = 0.0;
+  if (ctx->timing && (ctx->timing_label.empty() || ctx->timing_label == "ls_extrap_stage")) {
+    int h = 0;
+    EMX_TRY_HIP(hipMemcpyAsync(&h, count, sizeof(int), hipMemcpyDeviceToHost, st));
+    EMX_TRY_HIP(hipStreamSynchronize(st));
+    nsolid = h;
+  }
+  const unsigned bl = bg < 4096u ? bg : 4096u;
   for (int it = 1; it <= 10; it++) {
-    EMX_TIMED(ctx, "ls_extrap_stage", 72.0 * NG,
-          hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 0, P, P,
-                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D1));
-    EMX_TIMED(ctx, "ls_extrap_stage", 80.0 * NG,
-          hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 1, D1, P,
-                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D2));
-    EMX_TIMED(ctx, "ls_extrap_stage", 80.0 * NG,
-          hipLaunchKernelGGL(k_ls_extrap_stage, dim3(bg), dim3(256), 0, st, 2, D2, P,
-                       S, J1, J5, J9, dphi, g, iX, iY, iZ, dt, P));
+    EMX_TIMED(ctx, "ls_extrap_stage", (4.0 + 8.0 * 8) * nsolid,
+              hipLaunchKernelGGL(k_ls_extrap_list, dim3(bl), dim3(256), 0, st, 0, list, count, P,
+                                 P, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D1));
+    EMX_TIMED(ctx, "ls_extrap_stage", (4.0 + 8.0 * 9) * nsolid,
+              hipLaunchKernelGGL(k_ls_extrap_list, dim3(bl), dim3(256), 0, st, 1, list, count, D1,
+                                 P, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D2));
+    EMX_TIMED(ctx, "ls_extrap_stage", (4.0 + 8.0 * 9) * nsolid,
+              hipLaunchKernelGGL(k_ls_extrap_list, dim3(bl), dim3(256), 0, st, 2, list, count, D2,
+                                 P, J1, J5, J9, dphi, g, iX, iY, iZ, dt, P));
   }
   EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
           hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,

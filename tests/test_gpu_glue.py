@@ -114,3 +114,32 @@ def test_device_substep_vs_oracle(oracle):
             "pres": normwise_rel(host(D["pres"]), pres), "temp": normwise_rel(host(D["temp"]), temp)}
     print("  sub-step", {k: f"{v:.1e}" for k, v in errs.items()}, flush=True)
     assert max(errs.values()) <= 1e-12, errs
+
+
+@pytest.mark.parametrize("radius", [0.0, 0.12, 2.0])
+def test_extrapolation_solid_sizes(oracle, radius):
+    """Pressure extrapolation over the solid-point list: no solid (radius 0:
+    nothing changes), a small central ball, and a solid that covers the
+    whole box, against the oracle."""
+    shape = (30, 26, 22)
+    nx, ny, nz = shape
+    f = fields.stage_fields(nx, ny, nz, 81, curvilinear=True, turbulent=False)
+    x = np.ascontiguousarray(f["x"].numpy())
+    jbn = oracle.jacobeano(2, x)
+    pres = np.ascontiguousarray(f["pres"].numpy())
+    xs = fields.grid_uniform(*shape).numpy()
+    sob = np.ascontiguousarray(radius - np.sqrt(((xs - 0.5) ** 2).sum(0)))
+    if radius == 0.0:
+        sob = -np.abs(sob) - 1e-3
+    dxyz = (1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    ref = oracle.ls_extrapolate(pres, jbn, sob, dxyz, min(dxyz))
+    op = Operator(*shape, "c6c")
+    p = dev(pres)
+    op.advect_level_set(p, dev(jbn), dev(sob), dxyz, min(dxyz))
+    got = host(p)
+    print(f"  radius {radius}: solid points {(sob > 0).sum()}, "
+          f"err {normwise_rel(got, ref):.1e}", flush=True)
+    assert normwise_rel(got, ref) <= 1e-12
+    assert np.array_equal(got[sob <= 0], pres[sob <= 0])
+    if radius > 0:
+        assert not np.array_equal(got[sob > 0], pres[sob > 0])
