@@ -143,3 +143,64 @@ def test_extrapolation_solid_sizes(oracle, radius):
     assert np.array_equal(got[sob <= 0], pres[sob <= 0])
     if radius > 0:
         assert not np.array_equal(got[sob > 0], pres[sob > 0])
+
+
+def test_substep_graph_replays_match_eager(oracle):
+    """A whole device sub-step captured in one HIP graph (torch.cuda.graph,
+    the library launching on the capture stream) and replayed three times,
+    with eager copies and synchronisations between the replays: every replay
+    equals the eager sub-step bit for bit."""
+    shape = (40, 36, 32)
+    nx, ny, nz = shape
+    f = fields.stage_fields(nx, ny, nz, 90, curvilinear=True, turbulent=True, device="cuda")
+    op = Operator(*shape, "c6c")
+    jbn = op.jacobeano(f["x"])
+    sgd, _, _ = op.inisgdm(f["x"])
+    c4 = 1.0 / fields.REYNOLDS
+    gen = torch.Generator(device="cuda").manual_seed(91)
+    um0 = f["um"] * (1.0 + 0.01 * torch.randn(f["um"].shape, dtype=torch.float64,
+                                               device="cuda", generator=gen))
+    mask = (torch.rand(f["pres"].shape, dtype=torch.float64, device="cuda",
+                       generator=gen) > 0.05).double()
+    xs = fields.grid_uniform(*shape, device=torch.device("cuda"))
+    sob = (0.2 - torch.sqrt(((xs - 0.5) ** 2).sum(0))).contiguous()
+    dxyz = (1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1))
+    init = {k: f[k].clone() for k in ("um", "u", "temp", "pres")}
+    st = {k: v.clone() for k, v in init.items()}
+    work = op.stage_buffers()
+    um1 = op._empty(5)
+
+    def run():
+        op.rk_substep(2, um0, st["um"], st["u"], st["temp"], st["pres"], jbn, sgd, c4, 2e-3, 0.3,
+                      mask, sobject=sob, dxyz=dxyz, deltamin=min(dxyz), work=work, um1=um1)
+
+    def reset():
+        for k, v in init.items():
+            st[k].copy_(v)
+
+    run()  # eager: builds every table and launch shape before the capture
+    torch.cuda.synchronize()
+    ref = {"um1": um1.clone(), **{k: st[k].clone() for k in ("u", "temp", "pres")}}
+    reset()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            run()
+    torch.cuda.current_stream().wait_stream(s)
+    for rep in range(3):
+        reset()
+        um1.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(um1, ref["um1"]), rep
+        for k in ("u", "temp", "pres"):
+            assert torch.equal(st[k], ref[k]), (rep, k)
+        # eager work between replays (the sub-step updates u, temp, pres in
+        # place, so from the initial state again)
+        reset()
+        run()
+        torch.cuda.synchronize()
+        assert torch.equal(um1, ref["um1"])
