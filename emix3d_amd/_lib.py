@@ -43,6 +43,7 @@ SIGNATURES = {
     "emx_dynsmag": ([_P, _P, _P, _P, _P, _P, _D, _P], _I),
     "emx_rk_update": ([_P, _I, _P, _P, _P, _P, _P, _D], _I),
     "emx_frontera_val": ([_P, _P], _I),
+    "emx_ls_reinit": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
     "emx_varc_to_varnc": ([_P, _P, _D, _P, _P, _P, _P], _I),
     "emx_varnc_to_varc": ([_P, _P, _P, _P, _D, _P, _P], _I),
     "emx_rk_clip": ([_P, _P, _P, _D, _P, _P, _P, _P], _I),

@@ -429,6 +429,13 @@ class Operator:
                   self._check(sobject), *map(float, dxyz), float(deltamin))
         return pres
 
+    def reinit_level_set(self, phi, jbn, dmin, dxyz, deltamin):
+        """Level-set reinitialisation cuReinitLsJB of phi (in place)."""
+        self._stream()
+        _lib.call("emx_ls_reinit", self._ctx, self._check(phi), self._check(jbn, 11, "jbn"),
+                  self._check(dmin), *map(float, dxyz), float(deltamin))
+        return phi
+
     def advect_weno5(self, phi, vel, jbn, dxyz, dt):
         """One SSP-RK3 WENO5 advection step of phi (in place)."""
         self._stream()

@@ -456,6 +456,19 @@ int emx_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const d
   return op_rk_update(ctx, sub, um1, um0, um, rs, rsv, dt);
 }
 
+int emx_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmin, double dx,
+                  double dy, double dz, double deltamin) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(phi);
+  CHECK_PTR(jbn);
+  CHECK_PTR(dmin);
+  if (!(dx > 0.0 && dy > 0.0 && dz > 0.0 && deltamin > 0.0)) {
+    set_error("emx_ls_reinit: spacings and deltamin must be positive");
+    return EMX_ERR_ARG;
+  }
+  return op_ls_reinit(ctx, phi, jbn, dmin, dx, dy, dz, deltamin);
+}
+
 int emx_frontera_val(emx_ctx* ctx, double* um1) {
   EMX_TRY(check_ctx(ctx));
   CHECK_PTR(um1);

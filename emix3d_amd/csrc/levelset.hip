@@ -267,11 +267,71 @@ __device__ __forceinline__ double bdiff(const double* phi, PDims g, int ax,
   return ih * (phi[id] - phi[id - sxy]);
 }
 
+// The JB WENO5 derivatives of src along axis c at padded point id, from the
+// backward differences of the point's axis neighbourhood:
+// PhiDevPlusParameterJB (lsTools.cu:587-673) -> "left",
+// PhiDevMinusParameterJB (:676-770) -> "right"; only the requested sides
+// are formed.  m is the metric of the axis at the point.
+__device__ __forceinline__ void weno_axis(const double* src, PDims g, int c, int x, int y,
+                                          int z, long long id, double ihc, double m,
+                                          bool want_left, bool want_right, double* left,
+                                          double* right) {
+  const int pos[3] = {x, y, z};
+  const int ext[3] = {g.x, g.y, g.z};
+  const int q = pos[c], n = ext[c];
+  const long long stride = c == 0 ? 1 : (c == 1 ? (long long)g.x : (long long)g.x * g.y);
+  // the backward differences (Dev1thO_Downwind, lsTools.cu:53-95) at
+  // offsets -2..+3 along the axis, each once; positions outside the padded
+  // grid are clamped and never used below
+  double bd[6];
+#pragma unroll
+  for (int o = 0; o < 6; o++) {
+    int t = q - 2 + o;
+    t = t < 0 ? 0 : (t > n - 1 ? n - 1 : t);
+    const long long it = id + (long long)(t - q) * stride;
+    bd[o] = (t == 0 ? ihc * (src[it + stride] - src[it]) : ihc * (src[it] - src[it - stride])) * m;
+  }
+  double b[5];
+  if (want_left) {
+    // left stencil (PhiDevPlusParameterJB, lsTools.cu:587-673): positions
+    // q-2..q+2 with the reference's substitutions at the edges
+    int l[5] = {q - 2, q - 1, q, q + 1, q + 2};
+    bool zquirk = false;
+    if (q == 0) { l[1] = q; l[0] = q + 1; }
+    if (q == 1) { l[0] = l[1]; }
+    if (c < 2) {
+      if (q == n - 1) { l[3] = q; l[4] = l[1]; }
+      if (q == n - 2) { l[4] = l[3]; }
+    } else {
+      if (q == n - 1) { l[3] = q; zquirk = true; }  // id_kp2 = id_jm1 (lsTools.cu:636)
+      if (q == n - 2) { l[4] = l[3]; }
+    }
+#pragma unroll
+    for (int t = 0; t < 5; t++) b[t] = bd[l[t] - q + 2];
+    if (zquirk) {
+      const int jm1y = y == 0 ? y : y - 1;  // where id_jm1 ends up
+      b[4] = bdiff(src, g, 2, x, jm1y, z, ihc) * m;
+    }
+    *left = weno5(b[0], b[1], b[2], b[3], b[4]);
+  }
+  if (want_right) {
+    // right stencil (PhiDevMinusParameterJB, lsTools.cu:676-770):
+    // positions q+3, q+2, q+1, q, q-1
+    int rr[5] = {q + 3, q + 2, q + 1, q, q - 1};
+    if (q == 0) { rr[4] = q; }
+    if (q == n - 1) { rr[2] = q; rr[1] = rr[4]; rr[0] = q - 2; }
+    if (q == n - 2) { rr[1] = rr[2]; rr[0] = q; }
+    if (q == n - 3) { rr[0] = rr[1]; }
+#pragma unroll
+    for (int t = 0; t < 5; t++) b[t] = bd[rr[t] - q + 2];
+    *right = weno5(b[0], b[1], b[2], b[3], b[4]);
+  }
+}
+
 // One fused SSP-RK3 stage of cuAdvectLsJB (lsTransportEc.cu:243-399):
-// backward differences, the two JB WENO derivatives (PhiDevPlusParameterJB
-// lsTools.cu:587-673 -> "left", PhiDevMinusParameterJB :676-770 ->
-// "right"), the upwind right-hand side advect_RS_WENO (:361-417) and the
-// stage update.  src is the stage input, P the step's initial phi.
+// backward differences, the upwind JB WENO derivative, the right-hand side
+// advect_RS_WENO (:361-417) and the stage update.  src is the stage input,
+// P the step's initial phi.
 __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                                                  const double* P, const double* vel,
                                                  const double* j1, const double* j5,
@@ -282,8 +342,6 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
   const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
   const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
             z = (int)(row / (unsigned)g.y);
-  const int pos[3] = {x, y, z};
-  const int ext[3] = {g.x, g.y, g.z};
   const double ih[3] = {iX, iY, iZ};
   const double met[3] = {j1[id], j5[id], j9[id]};
   double r[3];
@@ -298,54 +356,9 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
       r[c] = 0.0;
       continue;
     }
-    const int q = pos[c], n = ext[c];
-    const double m = met[c];
-    const long long stride = c == 0 ? 1 : (c == 1 ? (long long)g.x : (long long)g.x * g.y);
-    // the backward differences (Dev1thO_Downwind, lsTools.cu:53-95) at
-    // offsets -2..+3 along the axis, each once; positions outside the
-    // padded grid are clamped and never used below
-    double bd[6];
-#pragma unroll
-    for (int o = 0; o < 6; o++) {
-      int t = q - 2 + o;
-      t = t < 0 ? 0 : (t > n - 1 ? n - 1 : t);
-      const long long it = id + (long long)(t - q) * stride;
-      bd[o] = (t == 0 ? ih[c] * (src[it + stride] - src[it])
-                      : ih[c] * (src[it] - src[it - stride])) * m;
-    }
-    double b[5];
-    if (v > 0.0) {
-      // left stencil (PhiDevPlusParameterJB, lsTools.cu:587-673): positions
-      // q-2..q+2 with the reference's substitutions at the edges
-      int l[5] = {q - 2, q - 1, q, q + 1, q + 2};
-      bool zquirk = false;
-      if (q == 0) { l[1] = q; l[0] = q + 1; }
-      if (q == 1) { l[0] = l[1]; }
-      if (c < 2) {
-        if (q == n - 1) { l[3] = q; l[4] = l[1]; }
-        if (q == n - 2) { l[4] = l[3]; }
-      } else {
-        if (q == n - 1) { l[3] = q; zquirk = true; }  // id_kp2 = id_jm1 (lsTools.cu:636)
-        if (q == n - 2) { l[4] = l[3]; }
-      }
-#pragma unroll
-      for (int t = 0; t < 5; t++) b[t] = bd[l[t] - q + 2];
-      if (zquirk) {
-        const int jm1y = y == 0 ? y : y - 1;  // where id_jm1 ends up
-        b[4] = bdiff(src, g, 2, x, jm1y, z, iZ) * m;
-      }
-    } else {
-      // right stencil (PhiDevMinusParameterJB, lsTools.cu:676-770):
-      // positions q+3, q+2, q+1, q, q-1
-      int rr[5] = {q + 3, q + 2, q + 1, q, q - 1};
-      if (q == 0) { rr[4] = q; }
-      if (q == n - 1) { rr[2] = q; rr[1] = rr[4]; rr[0] = q - 2; }
-      if (q == n - 2) { rr[1] = rr[2]; rr[0] = q; }
-      if (q == n - 3) { rr[0] = rr[1]; }
-#pragma unroll
-      for (int t = 0; t < 5; t++) b[t] = bd[rr[t] - q + 2];
-    }
-    r[c] = v * weno5(b[0], b[1], b[2], b[3], b[4]);
+    double L = 0.0, R = 0.0;
+    weno_axis(src, g, c, x, y, z, id, ih[c], met[c], v > 0.0, v < 0.0, &L, &R);
+    r[c] = v * (v > 0.0 ? L : R);
   }
   const double rs = r[0] + r[1] + r[2];
   const double sv = src[id];
@@ -363,6 +376,73 @@ __global__ void k_weno_stage(int stage, const double* src, const double* P,
   for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
        id += (long long)gridDim.x * blockDim.x)
     weno_stage_point(stage, src, P, vel, j1, j5, j9, g, iX, iY, iZ, dt, out, id);
+}
+
+// ---------------------------------------------------------------------------
+// Level-set reinitialisation (cuReinitLsJB, lsTransportEc.cu:422-676).
+
+// Ghost layers of the padded phi refreshed from its interior
+// (cuGhostCellsMirror3d{Z,Y,X}, lsTools.cu:482-585, applied in that order
+// over the full padded ranges, which leaves every ghost point equal to the
+// interior point at its mirrored coordinates), and phi0 = phi.
+__global__ void k_ls_ghost_mirror(double* P, double* P0, PDims g, int nx, int ny, int nz) {
+  const long long NG = g.N();
+  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+       id < NG; id += (long long)gridDim.x * blockDim.x) {
+    const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
+    const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
+              z = (int)(row / (unsigned)g.y);
+    const long long src = (long long)(mirror(x, nx) + GC) +
+                          (long long)g.x * ((long long)(mirror(y, ny) + GC) +
+                                            (long long)g.y * (mirror(z, nz) + GC));
+    const double v = P[src];  // an interior point: never written here
+    if (src != id) P[id] = v;
+    P0[id] = v;
+  }
+}
+
+// One SSP-RK3 stage of the reinitialisation at every padded point: the two
+// JB WENO derivatives per axis, the Godunov |grad phi| by the sign of phi0
+// and rs = S(phi0)(|grad phi| - 1), S = phi0/sqrt(phi0^2 + dmin^2)
+// (reini_RS_WENOJB, lsTools.cu:824-900), then RunGK_{First,Second,Third}S
+// (extrapol.cu:155-212).  src is the stage input, P the iteration's phi;
+// out may alias P only in the last stage.
+__global__ void k_ls_reinit_stage(int stage, const double* src, const double* P,
+                                  const double* P0, const double* dmin, const double* j1,
+                                  const double* j5, const double* j9, PDims g, double iX,
+                                  double iY, double iZ, double dt, double* out) {
+  const long long NG = g.N();
+  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+       id < NG; id += (long long)gridDim.x * blockDim.x) {
+    const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
+    const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
+              z = (int)(row / (unsigned)g.y);
+    const double ih[3] = {iX, iY, iZ};
+    const double met[3] = {j1[id], j5[id], j9[id]};
+    const double p0 = P0[id];
+    const double ta = (double)(p0 > 0.0) - (double)(p0 < 0.0);
+    double r[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double L, R;
+      weno_axis(src, g, c, x, y, z, id, ih[c], met[c], true, true, &L, &R);
+      const double lp = L > 0.0 ? L : 0.0, rm = R < 0.0 ? R : 0.0;
+      double hi = lp * lp, lo = rm * rm;
+      r[c] = 0.5 * (ta + 1.0) * (hi > lo ? hi : lo);
+      const double rp = R > 0.0 ? R : 0.0, lm = L < 0.0 ? L : 0.0;
+      hi = rp * rp;
+      lo = lm * lm;
+      r[c] += 0.5 * fabs(ta - 1.0) * (hi > lo ? hi : lo);
+    }
+    const double grad_mod = sqrt(r[0] + r[1] + r[2]);
+    const double d = dmin[id];
+    const double so = p0 / sqrt(p0 * p0 + d * d);
+    const double rs = 1.0 * so * (grad_mod - 1.0);
+    const double sv = src[id];
+    if (stage == 0) out[id] = sv - dt * rs;
+    else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (sv - dt * rs);
+    else out[id] = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -560,6 +640,57 @@ int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
   EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
           hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
                      d.nx, d.ny, d.nz, pres));
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+int op_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmin,
+                 double dx, double dy, double dz, double deltamin) {
+  const Dims d = ctx->d;
+  const PDims g = padded(d);
+  const long long N = d.N(), NG = g.N();
+  double* P = ctx->ls_buf;
+  double* P0 = P + NG;
+  double* J1 = P0 + NG;
+  double* J5 = J1 + NG;
+  double* J9 = J5 + NG;
+  double* Dm = J9 + NG;
+  double* T1 = Dm + NG;
+  double* T2 = T1 + NG;
+  hipStream_t st = ctx->stream;
+  const unsigned bg = blocks_for(NG);
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+            hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, phi, d.nx, d.ny, d.nz, P, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+            hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, dmin, d.nx, d.ny, d.nz, Dm, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+            hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+            hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz,
+                               J5, g));
+  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
+            hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz,
+                               J9, g));
+  // Dev1thO_Downwind receives 1.0/deltaX (lsTransportEc.cu:487-495)
+  const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz, dt = 0.5 * deltamin;
+  for (int it = 1; it <= 10; it++) {
+    EMX_TIMED(ctx, "ls_ghost_mirror", 16.0 * NG,
+              hipLaunchKernelGGL(k_ls_ghost_mirror, dim3(bg), dim3(256), 0, st, P, P0, g, d.nx,
+                                 d.ny, d.nz));
+    // per point: src, three metrics, phi0, dmin, out (and P in stages 1, 2)
+    EMX_TIMED(ctx, "ls_reinit_stage", 56.0 * NG,
+              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 0, P, P, P0, Dm,
+                                 J1, J5, J9, g, iX, iY, iZ, dt, T1));
+    EMX_TIMED(ctx, "ls_reinit_stage", 64.0 * NG,
+              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, P0, Dm,
+                                 J1, J5, J9, g, iX, iY, iZ, dt, T2));
+    EMX_TIMED(ctx, "ls_reinit_stage", 64.0 * NG,
+              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, P0, Dm,
+                                 J1, J5, J9, g, iX, iY, iZ, dt, P));
+  }
+  EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
+            hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g, d.nx, d.ny,
+                               d.nz, phi));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

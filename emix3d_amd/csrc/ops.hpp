@@ -31,6 +31,8 @@ int op_metrics(emx_ctx* ctx, const double* x, double* jbn, int* bad_dev);
 int op_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const double* um,
                  const double* rs, const double* rsv, double dt);
 int op_frontera_val(emx_ctx* ctx, double* um1);
+int op_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmin,
+                 double dx, double dy, double dz, double deltamin);
 int op_varc_to_varnc(emx_ctx* ctx, const double* um1, double mach, double* u, double* pres,
                      double* temp, double* conc);
 int op_varnc_to_varc(emx_ctx* ctx, const double* u, const double* pres, const double* temp,

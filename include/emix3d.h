@@ -346,6 +346,17 @@ int emx_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
                         const double* jbn, double dx, double dy, double dz,
                         double dt);
 
+/* Level-set reinitialisation, cuReinitLsJB (src/LevelSet/lsTransportEc.cu:
+ * 422-676): on the 5-cell mirror padding, ten iterations of ghost
+ * mirroring (cuGhostCellsMirror3d{Z,Y,X}, lsTools.cu:482-585, over the
+ * full padded ranges), phi0 = phi, and three SSP-RK3 stages of
+ * phi_t = S(phi0)(1 - |grad phi|) with the JB WENO5 derivatives and the
+ * Godunov gradient of reini_RS_WENOJB (lsTools.cu:824-900); S = phi0 /
+ * sqrt(phi0^2 + dmin^2), dt = deltamin/2, dx..dz the computational spacings.
+ * phi in place; dmin a field (the reference's d_dmins). */
+int emx_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmin, double dx,
+                  double dy, double dz, double deltamin);
+
 /* ---- host-pointer drop-ins with the reference's bind(C) signatures ------ */
 
 /* solver/deriv_vel_cuda.f90:19-39 -> solver/cuDeriv.cu:529-540.

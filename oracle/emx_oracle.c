@@ -811,6 +811,136 @@ int orc_ls_advect_weno5(int nx, int ny, int nz, double *phi, const double *vel,
     return rc;
 }
 
+/* cuGhostCellsMirror3dZ, then Y, then X (lsTools.cu:482-585) on a padded
+ * array of Nx x Ny x Nz points with g ghost layers, over the full padded
+ * ranges: ghost layer g-1-i takes interior layer g+i, layer n+g+i takes
+ * n-1-i+g (n = interior points), direction factor 1. */
+static void ls_ghost_mirror(int Nx, int Ny, int Nz, int g, double *a)
+{
+    const int nx = Nx - 2 * g, ny = Ny - 2 * g, nz = Nz - 2 * g;
+    const long sxy = (long)Nx * Ny;
+    for (int y = 0; y < Ny; y++)
+        for (int x = 0; x < Nx; x++)
+            for (int i = 0; i < g; i++) {
+                a[x + (long)Nx * y + sxy * (g - i - 1)] = a[x + (long)Nx * y + sxy * (g + i)];
+                a[x + (long)Nx * y + sxy * (nz + i + g)] = a[x + (long)Nx * y + sxy * (nz - i - 1 + g)];
+            }
+    for (int z = 0; z < Nz; z++)
+        for (int x = 0; x < Nx; x++)
+            for (int i = 0; i < g; i++) {
+                a[x + (long)Nx * (g - i - 1) + sxy * z] = a[x + (long)Nx * (g + i) + sxy * z];
+                a[x + (long)Nx * (ny + i + g) + sxy * z] = a[x + (long)Nx * (ny - i - 1 + g) + sxy * z];
+            }
+    for (int z = 0; z < Nz; z++)
+        for (int y = 0; y < Ny; y++)
+            for (int i = 0; i < g; i++) {
+                a[(g - i - 1) + (long)Nx * y + sxy * z] = a[(g + i) + (long)Nx * y + sxy * z];
+                a[(nx + i + g) + (long)Nx * y + sxy * z] = a[(nx - i - 1 + g) + (long)Nx * y + sxy * z];
+            }
+}
+
+int orc_ls_ghost_mirror(int Nx, int Ny, int Nz, int g, double *a)
+{
+    if (Nx <= 2 * g || Ny <= 2 * g || Nz <= 2 * g) return -1;
+    ls_ghost_mirror(Nx, Ny, Nz, g, a);
+    return 0;
+}
+
+/* reini_RS_WENOJB (lsTools.cu:824-900): Godunov |grad phi| from the left
+ * (pl, PhiDevPlusParameterJB) and right (pr, PhiDevMinusParameterJB)
+ * derivatives by the sign of phi0, rs = S(phi0)*(|grad phi| - 1) with
+ * S = phi0/sqrt(phi0^2 + dmin^2). */
+static void ls_reinit_rhs(long NG, double *rs, const double *phi0, const double *dmin,
+                          const double *pl, const double *pr)
+{
+    for (long id = 0; id < NG; id++) {
+        const double ta = (double)(phi0[id] > 0.0) - (double)(phi0[id] < 0.0);
+        double r[3];
+        for (int c = 0; c < 3; c++) {
+            const double P = pl[id + c * NG], M = pr[id + c * NG];
+            const double pp = P > 0.0 ? P : 0.0, mm = M < 0.0 ? M : 0.0;
+            double phiMax = pp * pp, phiMin = mm * mm;
+            r[c] = 0.5 * (ta + 1.0) * (phiMax > phiMin ? phiMax : phiMin);
+            const double mp = M > 0.0 ? M : 0.0, pm = P < 0.0 ? P : 0.0;
+            phiMax = mp * mp;
+            phiMin = pm * pm;
+            r[c] += 0.5 * fabs(ta - 1.0) * (phiMax > phiMin ? phiMax : phiMin);
+        }
+        const double grad_mod = sqrt(r[0] + r[1] + r[2]);
+        const double so = phi0[id] / sqrt(phi0[id] * phi0[id] + dmin[id] * dmin[id]);
+        rs[id] = 1.0 * so * (grad_mod - 1.0);
+    }
+}
+
+/* cuReinitLsJB (src/LevelSet/lsTransportEc.cu:422-676) on a padded grid
+ * with g ghost layers: ten iterations of ghost mirroring, phi0 = phi and
+ * three SSP-RK3 stages (RunGK_*, extrapol.cu:155-212) with dt = deltamin/2. */
+int orc_ls_reinit_padded(int Nx, int Ny, int Nz, int g, double *phi, const double *jbn,
+                         const double *dmin, double dx, double dy, double dz, double deltamin)
+{
+    if (Nx <= 2 * g || Ny <= 2 * g || Nz <= 2 * g || Nx < 4 || Ny < 4 || Nz < 4) return -1;
+    long NG = (long)Nx * Ny * Nz;
+    double *dphi = (double *)malloc(sizeof(double) * 3 * (size_t)NG);
+    double *pl = (double *)malloc(sizeof(double) * 3 * (size_t)NG);
+    double *pr = (double *)malloc(sizeof(double) * 3 * (size_t)NG);
+    double *rs = (double *)malloc(sizeof(double) * (size_t)NG);
+    double *tmp = (double *)malloc(sizeof(double) * (size_t)NG);
+    double *phi0 = (double *)malloc(sizeof(double) * (size_t)NG);
+    if (!dphi || !pl || !pr || !rs || !tmp || !phi0) {
+        free(dphi); free(pl); free(pr); free(rs); free(tmp); free(phi0);
+        return -3;
+    }
+    const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz, dt = 0.5 * deltamin;
+    for (int it = 1; it <= 10; it++) {
+        ls_ghost_mirror(Nx, Ny, Nz, g, phi);
+        memcpy(phi0, phi, sizeof(double) * (size_t)NG);
+        ls_backward_diff(Nx, Ny, Nz, dphi, phi, iX, iY, iZ);
+        ls_weno_left(Nx, Ny, Nz, pl, dphi, jbn);
+        ls_weno_right(Nx, Ny, Nz, pr, dphi, jbn);
+        ls_reinit_rhs(NG, rs, phi0, dmin, pl, pr);
+        for (long p = 0; p < NG; p++) tmp[p] = phi[p] - dt * rs[p];
+        ls_backward_diff(Nx, Ny, Nz, dphi, tmp, iX, iY, iZ);
+        ls_weno_right(Nx, Ny, Nz, pr, dphi, jbn);
+        ls_weno_left(Nx, Ny, Nz, pl, dphi, jbn);
+        ls_reinit_rhs(NG, rs, phi0, dmin, pl, pr);
+        for (long p = 0; p < NG; p++) tmp[p] = 0.75 * phi[p] + 0.25 * (tmp[p] - dt * rs[p]);
+        ls_backward_diff(Nx, Ny, Nz, dphi, tmp, iX, iY, iZ);
+        ls_weno_right(Nx, Ny, Nz, pr, dphi, jbn);
+        ls_weno_left(Nx, Ny, Nz, pl, dphi, jbn);
+        ls_reinit_rhs(NG, rs, phi0, dmin, pl, pr);
+        for (long p = 0; p < NG; p++) phi[p] = (phi[p] + 2.0 * (tmp[p] - dt * rs[p])) / 3.0;
+    }
+    free(dphi); free(pl); free(pr); free(rs); free(tmp); free(phi0);
+    return 0;
+}
+
+int orc_ls_reinit(int nx, int ny, int nz, double *phi, const double *jbn, const double *dmin,
+                  double dx, double dy, double dz, double deltamin)
+{
+    const int g = 5;
+    if (nx < 1 || ny < 1 || nz < 1) return -1;
+    int gx = nx + 2 * g, gy = ny + 2 * g, gz = nz + 2 * g;
+    long N = (long)nx * ny * nz, NG = (long)gx * gy * gz;
+    double *P = (double *)malloc(sizeof(double) * (size_t)NG);
+    double *D = (double *)malloc(sizeof(double) * (size_t)NG);
+    double *JB = (double *)calloc((size_t)(11 * NG), sizeof(double));
+    if (!P || !D || !JB) { free(P); free(D); free(JB); return -3; }
+    orc_pad_mirror(nx, ny, nz, g, phi, P);
+    orc_pad_mirror(nx, ny, nz, g, dmin, D);
+    orc_pad_mirror(nx, ny, nz, g, jbn, JB);
+    orc_pad_mirror(nx, ny, nz, g, jbn + 4 * N, JB + 4 * NG);
+    orc_pad_mirror(nx, ny, nz, g, jbn + 8 * N, JB + 8 * NG);
+    int rc = orc_ls_reinit_padded(gx, gy, gz, g, P, JB, D, dx, dy, dz, deltamin);
+    if (!rc)
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++)
+                    phi[IDX(i, j, k)] =
+                        P[(long)(i + g) + (long)gx * ((long)(j + g) + (long)gy * (k + g))];
+    free(P); free(D); free(JB);
+    return rc;
+}
+
 /* ---------------------------------------------------------------------- */
 /* Dynamic Smagorinsky (Germano 1991, Lilly 1992 least squares).           */
 

@@ -127,6 +127,16 @@ void orc_pad_mirror(int nx, int ny, int nz, int g, const double *in,
 int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gphys,
                 const double *delta, double *nut);
 
+/* Level-set reinitialisation cuReinitLsJB (src/LevelSet/lsTransportEc.cu:
+ * 422-676): ten iterations of ghost mirroring (cuGhostCellsMirror3d{Z,Y,X},
+ * lsTools.cu:482-585), phi0 = phi and three SSP-RK3 stages of
+ * phi_t = S(phi0)(1 - |grad phi|) (reini_RS_WENOJB, lsTools.cu:824-900, with
+ * the WENO5 derivatives of the advection), dt = deltamin/2, on the 5-cell
+ * mirror-padded grid; dmin is the smoothing width of S. */
+int orc_ls_ghost_mirror(int Nx, int Ny, int Nz, int g, double *a);
+int orc_ls_reinit(int nx, int ny, int nz, double *phi, const double *jbn, const double *dmin,
+                  double dx, double dy, double dz, double deltamin);
+
 /* SSP-RK3 update of sub-step sub (1..3), solver/rungek_ico.f90:87-91,
  * 152-153, 216-219, on n values (um may be NULL for sub-step 1). */
 int orc_rk_update(int sub, long n, double *um1, const double *um0, const double *um,

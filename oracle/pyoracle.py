@@ -63,6 +63,8 @@ def lib():
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
             "orc_rk_update": [_I, _L, _P, _P, _P, _P, _P, _D],
             "orc_frontera_val": [_I, _I, _I, _P],
+            "orc_ls_ghost_mirror": [_I, _I, _I, _I, _P],
+            "orc_ls_reinit": [_I, _I, _I, _P, _P, _P, _D, _D, _D, _D],
             "orc_filter_coeffs": [_I, _I, _D, _P, _P, _P],
             "orc_filtronper": [_I, _P, _P, _P, _P, _D],
             "orc_filtrado": [_I, _I, _I, _I, _P, _P, _D, _D, _P, _P, _P],
@@ -353,6 +355,24 @@ def ls_extrapolate(pres, jbn, sobject, dxyz, deltamin):
     out = np.array(pres, dtype=np.float64, copy=True)
     _check(lib().orc_ls_extrapolate(nx, ny, nz, _ptr(out), _ptr(jbn), _ptr(sobject),
                                     dxyz[0], dxyz[1], dxyz[2], deltamin), "ls_extrapolate")
+    return out
+
+
+def ls_ghost_mirror(padded, g=5):
+    """Ghost layers of a padded array (nzG, nyG, nxG) refreshed from its
+    interior (Z, then Y, then X mirror), on a copy."""
+    out = np.array(padded, dtype=np.float64, copy=True)
+    nzg, nyg, nxg = out.shape
+    _check(lib().orc_ls_ghost_mirror(nxg, nyg, nzg, g, _ptr(out)), "ls_ghost_mirror")
+    return out
+
+
+def ls_reinit(phi, jbn, dmin, dxyz, deltamin):
+    nx, ny, nz = dims(phi)
+    out = np.array(phi, dtype=np.float64, copy=True)
+    _check(lib().orc_ls_reinit(nx, ny, nz, _ptr(out), _ptr(np.ascontiguousarray(jbn)),
+                               _ptr(np.ascontiguousarray(dmin)), dxyz[0], dxyz[1], dxyz[2],
+                               deltamin), "ls_reinit")
     return out
 
 
