@@ -1,6 +1,7 @@
 """Level-set reinitialisation (cuReinitLsJB) on the GPU against the oracle,
 which test_oracle_reinit.py pins to the mathematics; 1e-12 normwise
-relative.  At 128^3: the plane distance is a fixed point in the core."""
+relative.  At 128^3: the plane distance is a fixed point 20 cells from the
+faces."""
 import numpy as np
 import pytest
 import torch
@@ -47,10 +48,12 @@ def test_reinit_plane_fixed_point_128():
     p = phi.clone()
     h = 1.0 / (n - 1)
     op.reinit_level_set(p, jbn, torch.full_like(phi, h), (h, h, h), h)
-    core = (slice(11, -11),) * 3
+    # the faces' mirror kinks reach about 16 cells in at this size (the
+    # oracle gives 4e-9 at 11 cells, 6e-13 at 16, 5e-16 at 20)
+    core = (slice(20, -20),) * 3
     err = float((p - phi)[core].abs().max())
     print(f"  plane 128^3 core change: {err:.1e}", flush=True)
-    assert err < 2e-12
+    assert err < 1e-13
 
 
 def test_reinit_argument_checks():
