@@ -43,6 +43,15 @@ SIGNATURES = {
     "emx_dynsmag": ([_P, _P, _P, _P, _P, _P, _D, _P], _I),
     "emx_rk_update": ([_P, _I, _P, _P, _P, _P, _P, _D], _I),
     "emx_frontera_val": ([_P, _P], _I),
+    "emx_io_write_field": ([ctypes.c_char_p, _I, _I, _I, _I, _P, _P, _P], _I),
+    "emx_io_read_field": ([ctypes.c_char_p, _I, _I, _I, _P, _P, _P], _I),
+    "emx_io_field_dims": ([ctypes.c_char_p] + [ctypes.POINTER(_I)] * 4, _I),
+    "emx_io_write_grid": ([ctypes.c_char_p, _I, _I, _I, _I, _P], _I),
+    "emx_io_read_grid": ([ctypes.c_char_p, _I, _I, _I, _P], _I),
+    "emx_io_write_time": ([ctypes.c_char_p, _D, _I], _I),
+    "emx_io_read_time": ([ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(_I)], _I),
+    "emx_io_write_scalar": ([ctypes.c_char_p, _I, _I, _I, _I, _P], _I),
+    "emx_io_read_scalar": ([ctypes.c_char_p, _I, _I, _I, _I, _P], _I),
     "emx_ls_reinit": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
     "emx_varc_to_varnc": ([_P, _P, _D, _P, _P, _P, _P], _I),
     "emx_varnc_to_varc": ([_P, _P, _P, _P, _D, _P, _P], _I),

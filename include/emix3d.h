@@ -52,6 +52,7 @@ extern "C" {
 #define EMX_ERR_SIZE (-3)   /* grid too small / too large for the scheme  */
 #define EMX_ERR_METRIC (-4) /* degenerate metric (|J| <= 1e-20)           */
 #define EMX_ERR_NOMEM (-5)  /* device allocation failed                   */
+#define EMX_ERR_IO (-6)     /* file missing, unreadable or malformed      */
 
 typedef struct emx_ctx emx_ctx;
 
@@ -356,6 +357,37 @@ int emx_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
  * phi in place; dmin a field (the reference's d_dmins). */
 int emx_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmin, double dx,
                   double dy, double dz, double deltamin);
+
+/* ---- the reference's on-disk formats (host memory, no device use) -------
+ * Fortran unformatted sequential files: each record a 4-byte length, the
+ * data, the length again; 4-byte integers, 8-byte reals, arrays
+ * column-major.  Records above 2^31-1 bytes (the compilers' continued
+ * records) are refused: EMX_ERR_SIZE on write, EMX_ERR_IO on read.
+ * Malformed or missing files give EMX_ERR_IO, header mismatches
+ * EMX_ERR_SIZE. */
+/* field_SS.NNN of GRAVA (solver/gravar_incomp.f90:20-35), read by
+ * LEER_CAMPOS (solver/leer_incomp.f90:45-96): (nd) (nx,ny,nz) u1 u2 u3 temp
+ * pres.  Reading requires nd = 5 and the given dimensions. */
+int emx_io_write_field(const char* path, int nd, int nx, int ny, int nz, const double* u,
+                       const double* temp, const double* pres);
+int emx_io_read_field(const char* path, int nx, int ny, int nz, double* u, double* temp,
+                      double* pres);
+int emx_io_field_dims(const char* path, int* nd, int* nx, int* ny, int* nz);
+/* The grid file of LEER_MALLA (leer_incomp.f90:1-41): (nvar) (nx,ny,nz)
+ * x y z; nvar is written as given and not checked on reading. */
+int emx_io_write_grid(const char* path, int nvar, int nx, int ny, int nz, const double* x);
+int emx_io_read_grid(const char* path, int nx, int ny, int nz, double* x);
+/* time_SS.NNN (gravar_incomp.f90:37-41, LEER_TIME leer_incomp.f90:245-262):
+ * one record (dto, it). */
+int emx_io_write_time(const char* path, double dto, int it);
+int emx_io_read_time(const char* path, double* dto, int* it);
+/* One scalar field: header 0 = the field record alone (impellermask_SS.NNN,
+ * gravar_incomp.f90:43-47); 1 = (nx,ny,nz) then the field (sobject.field,
+ * leer_incomp.f90:98-105); 2 = (count) (nx,ny,nz) then the field (mez.mask
+ * of LEER_MASCARA, leer_incomp.f90:215-232; count written as 1, ignored on
+ * reading). */
+int emx_io_write_scalar(const char* path, int header, int nx, int ny, int nz, const double* a);
+int emx_io_read_scalar(const char* path, int header, int nx, int ny, int nz, double* a);
 
 /* ---- host-pointer drop-ins with the reference's bind(C) signatures ------ */
 

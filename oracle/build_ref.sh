@@ -40,6 +40,8 @@ for scheme in c4 c6f; do
     "$REF/src/MSVSingle/filtro_incomp.f90"
     "$REF/src/MSVSingle/frontera_y.f90"
     "$REF/src/MSVSingle/frontera_z.f90"
+    "$REF/solver/gravar_incomp.f90"
+    "$REF/solver/leer_incomp.f90"
     "$HERE/ref_driver.f90"
   )
   OBJS=()

@@ -488,6 +488,45 @@ class RefFortran:
                             _ptr(pres))
         return out, u, pres, temp
 
+    def grava(self, u, temp, pres, mask, dto, it, serie, sample):
+        """GRAVA in the current working directory: field_SS.NNN,
+        time_SS.NNN, impellermask_SS.NNN."""
+        self.init(*dims(u))
+        self.L.ref_grava(_ptr(np.ascontiguousarray(u)), _ptr(np.ascontiguousarray(temp)),
+                         _ptr(np.ascontiguousarray(pres)), _ptr(np.ascontiguousarray(mask)),
+                         ctypes.c_double(dto), _I(it), _I(serie), _I(sample))
+        self.L.ref_flush()
+
+    def leer_campos(self, fname, mach, shape):
+        """LEER_CAMPOS of fname (and sobject.field) in the current working
+        directory; returns u, temp, pres, um0 (VARNC_TO_VARC), sobject."""
+        nx, ny, nz = shape
+        self.init(nx, ny, nz)
+        u = np.empty((3, nz, ny, nx))
+        temp, pres, sob = (np.empty((nz, ny, nx)) for _ in range(3))
+        um0 = np.empty((5, nz, ny, nx))
+        b = fname.encode()
+        self.L.ref_leer_campos(ctypes.c_char_p(b), _I(len(b)), ctypes.c_double(mach), _ptr(u),
+                               _ptr(temp), _ptr(pres), _ptr(um0), _ptr(sob))
+        self.L.ref_flush()
+        return u, temp, pres, um0, sob
+
+    def leer_malla(self, fname, shape):
+        nx, ny, nz = shape
+        self.init(nx, ny, nz)
+        x = np.empty((3, nz, ny, nx))
+        b = fname.encode()
+        self.L.ref_leer_malla(ctypes.c_char_p(b), _I(len(b)), _ptr(x))
+        self.L.ref_flush()
+        return x
+
+    def leer_time(self, fname):
+        dto, it = ctypes.c_double(), ctypes.c_int()
+        b = fname.encode()
+        self.L.ref_leer_time(ctypes.c_char_p(b), _I(len(b)), ctypes.byref(dto), ctypes.byref(it))
+        self.L.ref_flush()
+        return dto.value, it.value
+
     def jacobeano(self, x):
         self.init(*dims(x))
         jbn = np.empty((11,) + x.shape[1:])

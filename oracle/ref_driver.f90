@@ -475,3 +475,117 @@ subroutine ref_filtrado(itf, umio, jin, cfiltin, machin, uout, tout, pout) &
   pout = pres
   deallocate(um1)
 end subroutine ref_filtrado
+
+! ---- on-disk formats (run in the caller's working directory) ---------------
+
+! GRAVA (solver/gravar_incomp.f90:2-53): writes field_SS.NNN, time_SS.NNN
+! and impellermask_SS.NNN for serie SS = n_serie, sample NNN = i_sample.
+subroutine ref_grava(uin, tin, pin, maskin, dtoin, itin, serie_in, sample_in) &
+    bind(C, name='ref_grava')
+  use iso_c_binding
+  use dimensiones
+  use velocidades
+  use fieldrec
+  use tiempo
+  use maskvar
+  implicit none
+  real(c_double), intent(in) :: uin(nx,ny,nz,3), tin(nx,ny,nz), pin(nx,ny,nz)
+  real(c_double), intent(in) :: maskin(nx,ny,nz)
+  real(c_double), value :: dtoin
+  integer(c_int), value :: itin, serie_in, sample_in
+  if (allocated(impellermask)) deallocate(impellermask)
+  allocate(impellermask(nx,ny,nz))
+  u = uin
+  temp = tin
+  pres = pin
+  impellermask = maskin
+  dto = dtoin
+  it = itin
+  n_serie = serie_in
+  i_sample = sample_in
+  dgrava1 = 1.0
+  call grava()
+  deallocate(impellermask)
+end subroutine ref_grava
+
+! LEER_CAMPOS (solver/leer_incomp.f90:45-116): reads the field file named
+! fname and sobject.field, then VARNC_TO_VARC.
+subroutine ref_leer_campos(fname, n, machin, uout, tout, pout, umout, sout) &
+    bind(C, name='ref_leer_campos')
+  use iso_c_binding
+  use dimensiones
+  use velocidades
+  use variables
+  use inputname
+  use multiphase
+  use flow
+  implicit none
+  integer(c_int), value :: n
+  character(kind=c_char), intent(in) :: fname(n)
+  real(c_double), value :: machin
+  real(c_double), intent(out) :: uout(nx,ny,nz,3), tout(nx,ny,nz), pout(nx,ny,nz)
+  real(c_double), intent(out) :: umout(nx,ny,nz,5), sout(nx,ny,nz)
+  integer :: i
+  inputfield = ' '
+  do i = 1, n
+    inputfield(i:i) = fname(i)
+  end do
+  mach = machin
+  if (allocated(phi)) deallocate(phi)
+  if (allocated(dmins)) deallocate(dmins)
+  if (allocated(hevi)) deallocate(hevi)
+  if (allocated(rhoTwoPhase)) deallocate(rhoTwoPhase)
+  if (allocated(muTwoPhase)) deallocate(muTwoPhase)
+  if (allocated(rhoint)) deallocate(rhoint)
+  if (allocated(muint)) deallocate(muint)
+  if (allocated(sobject)) deallocate(sobject)
+  if (allocated(um0)) deallocate(um0)
+  if (.not. allocated(pres0)) allocate(pres0(nx,ny,nz))
+  allocate(um0(nx,ny,nz,5))
+  call leer_campos()
+  uout = u
+  tout = temp
+  pout = pres
+  umout = um0
+  sout = sobject
+  deallocate(phi, hevi, rhoTwoPhase, muTwoPhase, rhoint, muint, sobject, um0)
+end subroutine ref_leer_campos
+
+! LEER_MALLA (solver/leer_incomp.f90:1-41): reads the grid file fname.
+subroutine ref_leer_malla(fname, n, xout) bind(C, name='ref_leer_malla')
+  use iso_c_binding
+  use dimensiones
+  use mallagrid
+  use inputname
+  implicit none
+  integer(c_int), value :: n
+  character(kind=c_char), intent(in) :: fname(n)
+  real(c_double), intent(out) :: xout(nx,ny,nz,3)
+  integer :: i
+  inputgrid = ' '
+  do i = 1, n
+    inputgrid(i:i) = fname(i)
+  end do
+  call leer_malla()
+  xout = x
+end subroutine ref_leer_malla
+
+! LEER_TIME (solver/leer_incomp.f90:245-262).
+subroutine ref_leer_time(fname, n, dtoout, itout) bind(C, name='ref_leer_time')
+  use iso_c_binding
+  use inputname
+  use tiempo
+  implicit none
+  integer(c_int), value :: n
+  character(kind=c_char), intent(in) :: fname(n)
+  real(c_double), intent(out) :: dtoout
+  integer(c_int), intent(out) :: itout
+  integer :: i
+  inputtime = ' '
+  do i = 1, n
+    inputtime(i:i) = fname(i)
+  end do
+  call leer_time()
+  dtoout = dto
+  itout = it
+end subroutine ref_leer_time

@@ -8,6 +8,7 @@ container only (the reference tree is not shipped):
     python tests/golden/make_golden.py          # every fixture
     python tests/golden/make_golden.py glue     # the RUNGEK glue fixture only
     python tests/golden/make_golden.py filter   # the compact filter fixture only
+    python tests/golden/make_golden.py io       # the GRAVA files only
 """
 import os
 import sys
@@ -130,7 +131,40 @@ def make_filter():
     return out
 
 
+def make_io():
+    """Files written by the reference's GRAVA (field_03.007, time_03.007,
+    impellermask_03.007) for a seeded 7x6x5 state, copied next to this
+    script under io/, with the inputs in ref_io.npz."""
+    import shutil
+    import tempfile
+    ref = RefFortran("c6f")
+    nx, ny, nz = 7, 6, 5
+    rng = np.random.default_rng(20261213)
+    u = rng.standard_normal((3, nz, ny, nx))
+    temp = rng.standard_normal((nz, ny, nx))
+    pres = 1.0 + rng.random((nz, ny, nx))
+    mask = (rng.random((nz, ny, nx)) > 0.3).astype(np.float64)
+    dto, it = 12.625, 4321
+    out_dir = os.path.join(HERE, "io")
+    os.makedirs(out_dir, exist_ok=True)
+    cwd = os.getcwd()
+    with tempfile.TemporaryDirectory() as tmp:
+        os.chdir(tmp)
+        try:
+            ref.grava(u, temp, pres, mask, dto, it, 3, 7)
+        finally:
+            os.chdir(cwd)
+        for name in ("field_03.007", "time_03.007", "impellermask_03.007"):
+            shutil.copy(os.path.join(tmp, name), os.path.join(out_dir, name))
+    np.savez_compressed(os.path.join(HERE, "ref_io.npz"), u=u, temp=temp, pres=pres, mask=mask,
+                        dto=np.array(dto), it=np.array(it))
+
+
 def main():
+    if sys.argv[1:] == ["io"]:
+        make_io()
+        print("io files written to", HERE)
+        return
     if sys.argv[1:] == ["glue"]:
         np.savez_compressed(os.path.join(HERE, "ref_glue_g9x8x7.npz"), **make_glue())
         print("glue vectors written to", HERE)
@@ -141,6 +175,7 @@ def main():
         return
     np.savez_compressed(os.path.join(HERE, "ref_glue_g9x8x7.npz"), **make_glue())
     np.savez_compressed(os.path.join(HERE, "ref_filter.npz"), **make_filter())
+    make_io()
     for scheme in ("c4", "c6f"):
         if not RefFortran.available(scheme):
             raise SystemExit("oracle/_ref is not built: run `make -C oracle ref`")
