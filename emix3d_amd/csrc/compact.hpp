@@ -141,6 +141,132 @@ __device__ __forceinline__ int line_group(int bxp, int gx) {
   }
 }
 
+// Thomas factors and carry multipliers of the wavefront kind, staged in
+// LDS in the lanes' chunk order (see lines_wave_body).
+struct WaveTabs {
+  const double *ib, *m, *gam1, *H, *G, *af, *ab;
+};
+
+// Stages the wavefront kind's tables into lds (5*64*C + 768 doubles) in the
+// lanes' chunk order.  blob: ib[n] m[n] gamma[n+1] H[n] G[n]; table tb of
+// point l at tb*n + (tb >= 2) + l (gamma shifted by one), then Af, Ab.
+template <int C>
+__device__ __forceinline__ void stage_wave_tabs(const LineTab& T, double* lds) {
+  constexpr int TS = 64 * C;
+  const int n = T.n;
+  for (int t = threadIdx.x; t < 5 * TS + 768; t += blockDim.x) {
+    double val;
+    if (t < 5 * TS) {
+      const int tb = t / TS, r = t % TS, l = (r % 64) * C + r / 64;
+      val = l < n ? T.blob[tb * n + (tb >= 2 ? 1 : 0) + l] : 0.0;
+    } else {
+      val = T.Af[t - 5 * TS];
+    }
+    lds[t] = val;
+  }
+}
+
+// Solves NV fields of one line held by a wavefront in the coalesced order
+// (v[q][a]: point q*64 + lane of field a), in place: each field moves to
+// the chunk order through the wave's LDS buffer xb, takes the local
+// forward and backward sweeps and the carry scans, and moves back.
+template <int S, int C, int NV>
+__device__ __forceinline__ void wave_solve(double (&v)[C][NV], const WaveTabs& wt, double* xb,
+                                           int n, int P, double ih, int lane) {
+  constexpr int H = scheme_halo<S>::value;
+  constexpr int W = C + 2 * H;
+  const int s = lane * C;
+  const bool act = lane < P;
+#pragma unroll
+  for (int a = 0; a < NV; a++) {
+    // to the chunk order
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int idx = q * 64 + lane;
+      xb[idx + idx / C] = v[q][a];
+    }
+    wave_sync();
+    double w[W];
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+      const int l = clampi(s - H + q, 0, n - 1);
+      w[q] = xb[l + l / C];
+    }
+    // local forward sweep from a zero carry
+    double y[C];
+    double yp = 0.0;
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int l = s + q;
+      if (act && l < n) {
+        const double r = line_rhs<S>(&w[q + H], l, n, ih);
+        yp = r * wt.ib[q * 64 + lane] - wt.m[q * 64 + lane] * yp;
+        y[q] = yp;
+      } else {
+        y[q] = 0.0;
+      }
+    }
+    // forward carries: inclusive weighted scan over lanes
+    {
+      double B = act ? yp : 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        const int d = 1 << k;
+        const double up = __shfl_up(B, d, 64);
+        if (lane >= d) B += wt.af[k * 64 + lane] * up;
+      }
+      double Y = __shfl_up(B, 1, 64);
+      if (lane == 0) Y = 0.0;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (act && l < n) y[q] += wt.H[q * 64 + lane] * Y;
+      }
+    }
+    // local backward sweep from a zero carry
+    double xs = 0.0;
+#pragma unroll
+    for (int q = C - 1; q >= 0; q--) {
+      const int l = s + q;
+      if (act && l < n) {
+        xs = y[q] - wt.gam1[q * 64 + lane] * xs;
+        y[q] = xs;
+      }
+    }
+    // backward carries: weighted scan from the right, then the result
+    {
+      double B = act ? xs : 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        const int d = 1 << k;
+        const double dn = __shfl_down(B, d, 64);
+        if (lane + d < 64) B += wt.ab[k * 64 + lane] * dn;
+      }
+      double X = __shfl_down(B, 1, 64);
+      if (lane == 63) X = 0.0;
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        const int l = s + q;
+        if (act && l < n) y[q] += wt.G[q * 64 + lane] * X;
+      }
+    }
+    // back to the coalesced order
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int l = s + q;
+      if (act && l < n) xb[l + l / C] = y[q];
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int idx = q * 64 + lane;
+      v[q][a] = xb[idx + idx / C];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // One wavefront per line (xi lines, contiguous in memory).  Lines are
 // numbered L = j + nyl*k; base(L) = (L % nyl)*sj + (L / nyl)*sk, point l at
@@ -160,8 +286,6 @@ __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab&
                                                 long long blk, long long nblk) {
   constexpr int NV = Prob::NV;
   constexpr bool PRE = prefetches<Prob>::value;
-  constexpr int H = scheme_halo<S>::value;
-  constexpr int W = C + 2 * H;
   constexpr int STR = C + 1;  // chunk stride in the transpose buffer
   constexpr int TS = 64 * C;  // per-point table stride (chunk order)
   extern __shared__ double lds[];
@@ -173,6 +297,7 @@ __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab&
   double* s_G = s_H + TS;
   double* s_af = s_G + TS;    // Af[384] Ab[384]
   double* s_ab = s_af + 384;
+  const WaveTabs wt{s_ib, s_m, s_gam1, s_H, s_G, s_af, s_ab};
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double* xb = s_ab + 384 + wv * 64 * STR;
   const long long stride = (long long)nblk * 4;
@@ -190,23 +315,10 @@ __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab&
   };
   double v[C][NV];
   if (L < nlines) load_line(L, v);
-  // blob: ib[n] m[n] gamma[n+1] H[n] G[n]; table tb of point l at
-  // tb*n + (tb >= 2) + l (gamma shifted by one)
-  for (int t = threadIdx.x; t < 5 * TS + 768; t += 256) {
-    double val;
-    if (t < 5 * TS) {
-      const int tb = t / TS, r = t % TS, l = (r % 64) * C + r / 64;
-      val = l < n ? T.blob[tb * n + (tb >= 2 ? 1 : 0) + l] : 0.0;
-    } else {
-      val = T.Af[t - 5 * TS];
-    }
-    lds[t] = val;
-  }
+  stage_wave_tabs<C>(T, lds);
   __syncthreads();  // the tables; past here each wave only touches its own xb
   if (L >= nlines) return;
 
-  const int s = lane * C;
-  const bool act = lane < P;
   // The wave's lines L, L + stride, ...; with PRE the next line's loads are
   // in flight while the current one is solved.
   for (;;) {
@@ -216,94 +328,7 @@ __device__ __forceinline__ void lines_wave_body(const Prob& prob, const LineTab&
     if constexpr (PRE) {
       if (more) load_line(Ln, vn);
     }
-#pragma unroll
-    for (int a = 0; a < NV; a++) {
-      // to the chunk order
-      wave_sync();
-#pragma unroll
-      for (int q = 0; q < C; q++) {
-        const int idx = q * 64 + lane;
-        xb[idx + idx / C] = v[q][a];
-      }
-      wave_sync();
-      double w[W];
-#pragma unroll
-      for (int q = 0; q < W; q++) {
-        const int l = clampi(s - H + q, 0, n - 1);
-        w[q] = xb[l + l / C];
-      }
-      // local forward sweep from a zero carry
-      double y[C];
-      double yp = 0.0;
-#pragma unroll
-      for (int q = 0; q < C; q++) {
-        const int l = s + q;
-        if (act && l < n) {
-          const double r = line_rhs<S>(&w[q + H], l, n, ih);
-          yp = r * s_ib[q * 64 + lane] - s_m[q * 64 + lane] * yp;
-          y[q] = yp;
-        } else {
-          y[q] = 0.0;
-        }
-      }
-      // forward carries: inclusive weighted scan over lanes
-      {
-        double B = act ? yp : 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-          const int d = 1 << k;
-          const double up = __shfl_up(B, d, 64);
-          if (lane >= d) B += s_af[k * 64 + lane] * up;
-        }
-        double Y = __shfl_up(B, 1, 64);
-        if (lane == 0) Y = 0.0;
-#pragma unroll
-        for (int q = 0; q < C; q++) {
-          const int l = s + q;
-          if (act && l < n) y[q] += s_H[q * 64 + lane] * Y;
-        }
-      }
-      // local backward sweep from a zero carry
-      double xs = 0.0;
-#pragma unroll
-      for (int q = C - 1; q >= 0; q--) {
-        const int l = s + q;
-        if (act && l < n) {
-          xs = y[q] - s_gam1[q * 64 + lane] * xs;
-          y[q] = xs;
-        }
-      }
-      // backward carries: weighted scan from the right, then the result
-      {
-        double B = act ? xs : 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; k++) {
-          const int d = 1 << k;
-          const double dn = __shfl_down(B, d, 64);
-          if (lane + d < 64) B += s_ab[k * 64 + lane] * dn;
-        }
-        double X = __shfl_down(B, 1, 64);
-        if (lane == 63) X = 0.0;
-#pragma unroll
-        for (int q = 0; q < C; q++) {
-          const int l = s + q;
-          if (act && l < n) y[q] += s_G[q * 64 + lane] * X;
-        }
-      }
-      // back to the coalesced order
-      wave_sync();
-#pragma unroll
-      for (int q = 0; q < C; q++) {
-        const int l = s + q;
-        if (act && l < n) xb[l + l / C] = y[q];
-      }
-      wave_sync();
-#pragma unroll
-      for (int q = 0; q < C; q++) {
-        const int idx = q * 64 + lane;
-        v[q][a] = xb[idx + idx / C];
-      }
-    }
+    wave_solve<S, C, NV>(v, wt, xb, n, P, ih, lane);
     const long long bs = line_base(L);
 #pragma unroll
     for (int q = 0; q < C; q++) {

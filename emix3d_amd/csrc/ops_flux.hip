@@ -74,6 +74,121 @@ static int project_reads(int mask) {
   return __builtin_popcount(um) + 3 + 2 + 4 + __builtin_popcount(dv);
 }
 
+// The projection fused with the xi solves, for plans with xi on the
+// materialized kind and the wavefront kind for xi derivatives: one
+// workgroup per xi line (persistent over lines).  Its work-items form the
+// projected fluxes of their points of the line from the 22 inputs, as
+// k_flux_project: the eta/zeta ones (directions in yz, block k of eight
+// scratch fields for the k-th materialized direction of mat) go to scratch,
+// the eight xi ones to LDS.  Then each of the four wavefronts solves two of
+// the eight xi lines with the wavefront kind's solve and writes them as
+// the first-pass accumulators (ProbAccum pass 0).  The xi fluxes never go
+// to memory.
+template <int S, int C>
+__global__ void __launch_bounds__(256)
+k_flux_project_xi(FluxIn F, LineTab T, long long nlines, int nx, int ny, int yz, int mat,
+                  double ih, double* fluxbuf, double* acc) {
+  constexpr int TS = 64 * C, STR = C + 1;
+  extern __shared__ double lds[];
+  // tables | per-wave transpose buffers | the line's eight xi fluxes
+  const WaveTabs wt{lds, lds + TS, lds + 2 * TS, lds + 3 * TS, lds + 4 * TS, lds + 5 * TS,
+                    lds + 5 * TS + 384};
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double* xb = lds + 5 * TS + 768 + wv * 64 * STR;
+  double* lineF = lds + 5 * TS + 768 + 4 * 64 * STR;  // [8][TS]
+  const long long N = F.N;
+  const int n = T.n, P = T.P;
+  stage_wave_tabs<C>(T, lds);
+  for (long long L = blockIdx.x; L < nlines; L += gridDim.x) {
+    const long long base = (L % ny) * nx + (L / ny) * (long long)nx * ny;
+    for (int x = threadIdx.x; x < nx; x += blockDim.x) {
+      const long long p = base + x;
+      const FluxPoint Pt(F, p);
+      const double det = Pt.J(10, p);
+#pragma unroll
+      for (int dir = 0; dir < 3; dir++) {
+        if (dir > 0 && !(yz & (1 << dir))) continue;
+        const double met = Pt.J(1 + 4 * dir, p);
+        const int k = __builtin_popcount(mat & ((1 << dir) - 1));
+#pragma unroll
+        for (int slot = 0; slot < 8; slot++) {
+          const double f =
+              slot < 5 ? conv_flux(Pt, slot + 1, dir, p) : visc_flux(Pt, slot - 3, dir, p);
+          const double val = det * (met * f);
+          if (dir == 0) lineF[slot * TS + x] = val;
+          else fluxbuf[(8 * k + slot) * N + p] = val;
+        }
+      }
+    }
+    __syncthreads();  // the line's xi fluxes (and, the first time, the tables)
+    double v[C][2];
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int idx = q * 64 + lane;
+      const int idc = idx < n ? idx : n - 1;
+#pragma unroll
+      for (int a = 0; a < 2; a++) v[q][a] = lineF[(2 * wv + a) * TS + idc];
+    }
+    wave_solve<S, C, 2>(v, wt, xb, n, P, ih, lane);
+#pragma unroll
+    for (int q = 0; q < C; q++) {
+      const int idx = q * 64 + lane;
+      if (idx < n) {
+#pragma unroll
+        for (int a = 0; a < 2; a++) acc[(2 * wv + a) * N + base + idx] = v[q][a];
+      }
+    }
+    __syncthreads();  // lineF is rewritten for the next line
+  }
+}
+
+template <int S, int C>
+static hipError_t launch_project_xi_c(emx_ctx* ctx, const FluxIn& in, const LineTab& T, int yz,
+                                      int mat, double ih) {
+  const size_t lds = (size_t)(5 * 64 * C + 768 + 4 * 64 * (C + 1) + 8 * 64 * C) * sizeof(double);
+  auto kern = k_flux_project_xi<S, C>;
+  if (lds > 65536)
+    (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+  static const int resident = resident_blocks(kern, lds);
+  const long long nlines = (long long)ctx->d.ny * ctx->d.nz;
+  const long long blocks = nlines < resident ? nlines : resident;
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), lds, ctx->stream, in, T, nlines,
+                     ctx->d.nx, ctx->d.ny, yz, mat, ih, ctx->fluxbuf, ctx->acc);
+  return hipGetLastError();
+}
+
+template <int S>
+static hipError_t launch_project_xi_s(emx_ctx* ctx, const FluxIn& in, const LineTab& T, int yz,
+                                      int mat, double ih) {
+  switch (T.C) {
+    case 1: return launch_project_xi_c<S, 1>(ctx, in, T, yz, mat, ih);
+    case 2: return launch_project_xi_c<S, 2>(ctx, in, T, yz, mat, ih);
+    case 4: return launch_project_xi_c<S, 4>(ctx, in, T, yz, mat, ih);
+    case 8: return launch_project_xi_c<S, 8>(ctx, in, T, yz, mat, ih);
+    default: return launch_project_xi_c<S, 16>(ctx, in, T, yz, mat, ih);
+  }
+}
+
+static int launch_project_xi(emx_ctx* ctx, const FluxIn& in, int mat, double ih) {
+  const long long N = in.N;
+  const int yz = mat & 6;
+  const LineTab& T = ctx->plan[PLAN_DERIV][0].tab;
+  const double bytes = 8.0 * N * (project_reads(mat) + 8 * __builtin_popcount(yz) + 8);
+  LaunchTimer timer(ctx, "flux_project_xi", 0, bytes);
+  hipError_t e;
+  switch (ctx->scheme) {
+    case EMX_SCHEME_C4: e = launch_project_xi_s<EMX_SCHEME_C4>(ctx, in, T, yz, mat, ih); break;
+    case EMX_SCHEME_C6F: e = launch_project_xi_s<EMX_SCHEME_C6F>(ctx, in, T, yz, mat, ih); break;
+    default: e = launch_project_xi_s<EMX_SCHEME_C6C>(ctx, in, T, yz, mat, ih); break;
+  }
+  if (e != hipSuccess) {
+    set_error("fused projection and xi solve launch failed: %s", hipGetErrorString(e));
+    return EMX_ERR_HIP;
+  }
+  return EMX_OK;
+}
+
 static int launch_project(emx_ctx* ctx, const FluxIn& in, int mask) {
   const long long N = in.N;
   const double bytes = 8.0 * N * (project_reads(mask) + 8 * __builtin_popcount(mask));
@@ -165,9 +280,13 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
               ctx->fluxbuf_dirs, __builtin_popcount(mat));
     return EMX_ERR_NOMEM;
   }
-  EMX_TRY(launch_project(ctx, in, mat));
-  int pass = 0, k = 0;
-  for (int dir = 0; dir < 3; dir++) {
+  // xi materialized with the wavefront kind for its solves: the projection
+  // and the xi solves in one pass (the xi fluxes stay in LDS)
+  const bool fused_xi = (mat & 1) && ctx->plan[PLAN_DERIV][0].kind == KIND_WAVE;
+  if (fused_xi) EMX_TRY(launch_project_xi(ctx, in, mat, ih[0]));
+  else EMX_TRY(launch_project(ctx, in, mat));
+  int pass = fused_xi ? 1 : 0, k = fused_xi ? 1 : 0;
+  for (int dir = fused_xi ? 1 : 0; dir < 3; dir++) {
     if (!(mat & (1 << dir))) continue;
     // slots (0,1) (2,3) (4,5) (6,7): outputs rs 1..5 then rsv 2..4
     for (int slot = 0; slot < 8; slot += 2) {
