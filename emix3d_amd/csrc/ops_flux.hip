@@ -88,14 +88,16 @@ template <int S, int C>
 __global__ void __launch_bounds__(256)
 k_flux_project_xi(FluxIn F, LineTab T, long long nlines, int nx, int ny, int yz, int mat,
                   double ih, double* fluxbuf, double* acc) {
-  constexpr int TS = 64 * C, STR = C + 1;
+  constexpr int TS = 64 * C;
   extern __shared__ double lds[];
-  // tables | per-wave transpose buffers | the line's eight xi fluxes
+  // tables | the line's eight xi fluxes.  Wave w solves slots 2w, 2w+1 and
+  // uses their rows (2*TS >= 64*(C+1) doubles) as its transpose buffer once
+  // it has read them: no other wave touches them before the next line.
   const WaveTabs wt{lds, lds + TS, lds + 2 * TS, lds + 3 * TS, lds + 4 * TS, lds + 5 * TS,
                     lds + 5 * TS + 384};
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  double* xb = lds + 5 * TS + 768 + wv * 64 * STR;
-  double* lineF = lds + 5 * TS + 768 + 4 * 64 * STR;  // [8][TS]
+  double* lineF = lds + 5 * TS + 768;  // [8][TS]
+  double* xb = lineF + 2 * wv * TS;
   const long long N = F.N;
   const int n = T.n, P = T.P;
   stage_wave_tabs<C>(T, lds);
@@ -145,7 +147,7 @@ k_flux_project_xi(FluxIn F, LineTab T, long long nlines, int nx, int ny, int yz,
 template <int S, int C>
 static hipError_t launch_project_xi_c(emx_ctx* ctx, const FluxIn& in, const LineTab& T, int yz,
                                       int mat, double ih) {
-  const size_t lds = (size_t)(5 * 64 * C + 768 + 4 * 64 * (C + 1) + 8 * 64 * C) * sizeof(double);
+  const size_t lds = (size_t)(5 * 64 * C + 768 + 8 * 64 * C) * sizeof(double);
   auto kern = k_flux_project_xi<S, C>;
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
