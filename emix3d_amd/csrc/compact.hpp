@@ -124,6 +124,14 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroup barrier that orders LDS traffic only: global loads issued
+// before it stay in flight (a full __syncthreads also waits for them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Line group of workgroup bxp (of gx along the lines) for LG lines per
 // wavefront.  A group of 8 doubles (64 bytes) shares each 128-byte cache
 // line with its neighbour.  Workgroups go to the eight XCDs round robin,

@@ -88,6 +88,7 @@ struct FluxIn {
 // uses are dropped by the compiler.
 struct FluxPoint {
   double um[5], u[3], pres, vis, j[10], d[9];
+  __device__ __forceinline__ FluxPoint() {}
   __device__ __forceinline__ FluxPoint(const FluxIn& F, long long p) {
 #pragma unroll
     for (int m = 1; m < 5; m++) um[m] = F.UM(m + 1, p);

@@ -98,14 +98,20 @@ k_flux_project_xi(FluxIn F, LineTab T, long long nlines, int nx, int ny, int yz,
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double* lineF = lds + 5 * TS + 768;  // [8][TS]
   double* xb = lineF + 2 * wv * TS;
+  // lines of at most 256 points: one point per work-item, whose next-line
+  // inputs are loaded while the current line is solved
+  constexpr bool PRE = C <= 4;
   const long long N = F.N;
   const int n = T.n, P = T.P;
+  auto line_base = [&](long long Lx) { return (Lx % ny) * nx + (Lx / ny) * (long long)nx * ny; };
   stage_wave_tabs<C>(T, lds);
+  FluxPoint Pn;
+  if (PRE && blockIdx.x < nlines && threadIdx.x < nx) Pn = FluxPoint(F, line_base(blockIdx.x) + threadIdx.x);
   for (long long L = blockIdx.x; L < nlines; L += gridDim.x) {
-    const long long base = (L % ny) * nx + (L / ny) * (long long)nx * ny;
+    const long long base = line_base(L);
     for (int x = threadIdx.x; x < nx; x += blockDim.x) {
       const long long p = base + x;
-      const FluxPoint Pt(F, p);
+      const FluxPoint Pt = PRE ? Pn : FluxPoint(F, p);
       const double det = Pt.J(10, p);
 #pragma unroll
       for (int dir = 0; dir < 3; dir++) {
@@ -122,7 +128,9 @@ k_flux_project_xi(FluxIn F, LineTab T, long long nlines, int nx, int ny, int yz,
         }
       }
     }
-    __syncthreads();  // the line's xi fluxes (and, the first time, the tables)
+    if (PRE && L + gridDim.x < nlines && threadIdx.x < nx)
+      Pn = FluxPoint(F, line_base(L + gridDim.x) + threadIdx.x);
+    lds_barrier();  // the line's xi fluxes (and, the first time, the tables)
     double v[C][2];
 #pragma unroll
     for (int q = 0; q < C; q++) {
@@ -140,7 +148,7 @@ k_flux_project_xi(FluxIn F, LineTab T, long long nlines, int nx, int ny, int yz,
         for (int a = 0; a < 2; a++) acc[(2 * wv + a) * N + base + idx] = v[q][a];
       }
     }
-    __syncthreads();  // lineF is rewritten for the next line
+    lds_barrier();  // lineF is rewritten for the next line
   }
 }
 
