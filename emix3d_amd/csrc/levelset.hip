@@ -267,23 +267,13 @@ __device__ __forceinline__ double bdiff(const double* phi, PDims g, int ax,
   return ih * (phi[id] - phi[id - sxy]);
 }
 
-// The JB WENO5 derivatives of src along axis c at padded point id, from the
-// backward differences of the point's axis neighbourhood:
-// PhiDevPlusParameterJB (lsTools.cu:587-673) -> "left",
-// PhiDevMinusParameterJB (:676-770) -> "right"; only the requested sides
-// are formed.  m is the metric of the axis at the point.
-__device__ __forceinline__ void weno_axis(const double* src, PDims g, int c, int x, int y,
-                                          int z, long long id, double ihc, double m,
-                                          bool want_left, bool want_right, double* left,
-                                          double* right) {
-  const int pos[3] = {x, y, z};
-  const int ext[3] = {g.x, g.y, g.z};
-  const int q = pos[c], n = ext[c];
+// The backward differences (Dev1thO_Downwind, lsTools.cu:53-95) times the
+// metric m along axis c at offsets -2..+3 from padded point id (axis
+// position q of n), each once; positions outside the padded grid are
+// clamped and never used by the stencils below.
+__device__ __forceinline__ void axis_bd(const double* src, PDims g, int c, int q, int n,
+                                        long long id, double ihc, double m, double bd[6]) {
   const long long stride = c == 0 ? 1 : (c == 1 ? (long long)g.x : (long long)g.x * g.y);
-  // the backward differences (Dev1thO_Downwind, lsTools.cu:53-95) at
-  // offsets -2..+3 along the axis, each once; positions outside the padded
-  // grid are clamped and never used below
-  double bd[6];
 #pragma unroll
   for (int o = 0; o < 6; o++) {
     int t = q - 2 + o;
@@ -291,10 +281,16 @@ __device__ __forceinline__ void weno_axis(const double* src, PDims g, int c, int
     const long long it = id + (long long)(t - q) * stride;
     bd[o] = (t == 0 ? ihc * (src[it + stride] - src[it]) : ihc * (src[it] - src[it - stride])) * m;
   }
-  double b[5];
-  if (want_left) {
-    // left stencil (PhiDevPlusParameterJB, lsTools.cu:587-673): positions
-    // q-2..q+2 with the reference's substitutions at the edges
+}
+
+// The five WENO5 operands of one side along axis c from bd, with the
+// reference's substitutions at the edges: left = PhiDevPlusParameterJB
+// (lsTools.cu:587-673, positions q-2..q+2), right = PhiDevMinusParameterJB
+// (:676-770, positions q+3, q+2, q+1, q, q-1).
+__device__ __forceinline__ void axis_stencil(const double* src, PDims g, int c, int q, int n,
+                                             int x, int y, int z, double ihc, double m,
+                                             const double bd[6], bool left, double b[5]) {
+  if (left) {
     int l[5] = {q - 2, q - 1, q, q + 1, q + 2};
     bool zquirk = false;
     if (q == 0) { l[1] = q; l[0] = q + 1; }
@@ -312,11 +308,7 @@ __device__ __forceinline__ void weno_axis(const double* src, PDims g, int c, int
       const int jm1y = y == 0 ? y : y - 1;  // where id_jm1 ends up
       b[4] = bdiff(src, g, 2, x, jm1y, z, ihc) * m;
     }
-    *left = weno5(b[0], b[1], b[2], b[3], b[4]);
-  }
-  if (want_right) {
-    // right stencil (PhiDevMinusParameterJB, lsTools.cu:676-770):
-    // positions q+3, q+2, q+1, q, q-1
+  } else {
     int rr[5] = {q + 3, q + 2, q + 1, q, q - 1};
     if (q == 0) { rr[4] = q; }
     if (q == n - 1) { rr[2] = q; rr[1] = rr[4]; rr[0] = q - 2; }
@@ -324,7 +316,6 @@ __device__ __forceinline__ void weno_axis(const double* src, PDims g, int c, int
     if (q == n - 3) { rr[0] = rr[1]; }
 #pragma unroll
     for (int t = 0; t < 5; t++) b[t] = bd[rr[t] - q + 2];
-    *right = weno5(b[0], b[1], b[2], b[3], b[4]);
   }
 }
 
@@ -356,9 +347,11 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
       r[c] = 0.0;
       continue;
     }
-    double L = 0.0, R = 0.0;
-    weno_axis(src, g, c, x, y, z, id, ih[c], met[c], v > 0.0, v < 0.0, &L, &R);
-    r[c] = v * (v > 0.0 ? L : R);
+    const int pos[3] = {x, y, z}, ext[3] = {g.x, g.y, g.z};
+    double bd[6], b[5];
+    axis_bd(src, g, c, pos[c], ext[c], id, ih[c], met[c], bd);
+    axis_stencil(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, v > 0.0, b);
+    r[c] = v * weno5(b[0], b[1], b[2], b[3], b[4]);
   }
   const double rs = r[0] + r[1] + r[2];
   const double sv = src[id];
@@ -424,8 +417,13 @@ __global__ void k_ls_reinit_stage(int stage, const double* src, const double* P,
     double r[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      double L, R;
-      weno_axis(src, g, c, x, y, z, id, ih[c], met[c], true, true, &L, &R);
+      const int pos[3] = {x, y, z}, ext[3] = {g.x, g.y, g.z};
+      double bd[6], b[5];
+      axis_bd(src, g, c, pos[c], ext[c], id, ih[c], met[c], bd);
+      axis_stencil(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, true, b);
+      const double L = weno5(b[0], b[1], b[2], b[3], b[4]);
+      axis_stencil(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, false, b);
+      const double R = weno5(b[0], b[1], b[2], b[3], b[4]);
       const double lp = L > 0.0 ? L : 0.0, rm = R < 0.0 ? R : 0.0;
       double hi = lp * lp, lo = rm * rm;
       r[c] = 0.5 * (ta + 1.0) * (hi > lo ? hi : lo);
