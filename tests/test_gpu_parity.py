@@ -410,3 +410,24 @@ def test_timing_select_records_one_label():
     op.derivative(f, 1)
     op.rk_update(1, um, None, um, um, 0.1)
     assert sorted(e["label"] for e in op.timing_end()) == ["deriv", "rk_update"]
+
+
+@pytest.mark.parametrize("shape", [(40, 9, 8), (100, 7, 6), (130, 6, 5), (300, 6, 5),
+                                   (700, 5, 5)])
+def test_flux_div_all_long_xi_lines(oracle, shape):
+    """The projection fused with the xi solves for every xi chunk length the
+    wavefront kind takes (C = 1, 2, 4, 8, 16; the last needs more than 64 KB
+    of LDS and keeps no prefetch), against the oracle."""
+    z = stage_inputs(shape, 37, "c6c", oracle)
+    _, vis, _ = oracle.sgdm(z["u"], z["dvel"], z["jbn"], z["sgd"], z["c4"])
+    op = Operator(*shape, "c6c")
+    ih = op.default_ih()
+    D = {k: dev(z[k]) for k in ("um", "u", "pres", "jbn", "dvel")}
+    rs, rsv = op.flujos_divergencia_all(D["um"], D["u"], D["pres"], dev(vis), D["jbn"],
+                                        D["dvel"])
+    rs, rsv = host(rs), host(rsv)
+    for neq in range(1, 6):
+        flux = oracle.flujos(neq, z["um"], z["u"], z["pres"], vis, z["jbn"], z["dvel"])
+        rrs, rrsv = oracle.divergencia(2, flux, z["jbn"], ih)
+        report(f"{shape} rs{neq}", normwise_rel(rs[neq - 1], rrs))
+        report(f"{shape} rsv{neq}", normwise_rel(rsv[neq - 1], rrsv))
