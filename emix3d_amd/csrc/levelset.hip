@@ -236,13 +236,18 @@ __device__ __forceinline__ double weno5(double b1, double b2, double b3,
                     (0.25) * (b2 - b4) * (b2 - b4);
   const double s3 = (13.0 / 12.0) * (b3 - 2.0 * b4 + b5) * (b3 - 2.0 * b4 + b5) +
                     (0.25) * (3.0 * b3 - 4.0 * b4 + b5) * (3.0 * b3 - 4.0 * b4 + b5);
-  const double a1 = 0.1 / ((s1 + 1.0e-6) * (s1 + 1.0e-6));
-  const double a2 = 0.6 / ((s2 + 1.0e-6) * (s2 + 1.0e-6));
-  const double a3 = 0.3 / ((s3 + 1.0e-6) * (s3 + 1.0e-6));
-  const double isum = 1.0 / (a1 + a2 + a3);  // one division for the three weights
-  const double o1 = a1 * isum;
-  const double o2 = a2 * isum;
-  const double o3 = a3 * isum;
+  // weights a_k = c_k/d_k, d_k = (s_k + 1e-6)^2, normalised: o_k = c_k p_k /
+  // sum_j c_j p_j with p_k the product of the other two d (one division
+  // instead of four; d_k lies in [1e-12, ~1e16], so the products stay far
+  // from the double range's ends)
+  const double d1 = (s1 + 1.0e-6) * (s1 + 1.0e-6);
+  const double d2 = (s2 + 1.0e-6) * (s2 + 1.0e-6);
+  const double d3 = (s3 + 1.0e-6) * (s3 + 1.0e-6);
+  const double w1 = 0.1 * (d2 * d3), w2 = 0.6 * (d1 * d3), w3 = 0.3 * (d1 * d2);
+  const double isum = 1.0 / (w1 + w2 + w3);
+  const double o1 = w1 * isum;
+  const double o2 = w2 * isum;
+  const double o3 = w3 * isum;
   return ((o1 * (2.0 * b1 - 7.0 * b2 + 11.0 * b3) +
            o2 * (-1.0 * b2 + 5.0 * b3 + 2.0 * b4) +
            o3 * (2.0 * b3 + 5.0 * b4 - b5)) *
