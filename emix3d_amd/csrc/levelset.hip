@@ -273,55 +273,58 @@ __device__ __forceinline__ double bdiff(const double* phi, PDims g, int ax,
 }
 
 // The backward differences (Dev1thO_Downwind, lsTools.cu:53-95) times the
-// metric m along axis c at offsets -2..+3 from padded point id (axis
-// position q of n), each once; positions outside the padded grid are
-// clamped and never used by the stencils below.
-__device__ __forceinline__ void axis_bd(const double* src, PDims g, int c, int q, int n,
-                                        long long id, double ihc, double m, double bd[6]) {
-  const long long stride = c == 0 ? 1 : (c == 1 ? (long long)g.x : (long long)g.x * g.y);
+// metric m along axis c at offsets -2..+3 from padded point uid (axis
+// position q of n, index stride `stride`).  phi is read once at each of the
+// seven positions q-3..q+3, clamped into the grid (a clamped position's
+// difference is never used by the stencils below); at position 0 the
+// difference is the forward one, phi(1) - phi(0), which is the backward
+// difference at position 1.  Indices are 32-bit: the padded grid has fewer
+// than 2^32 points, so the wrapped unsigned sum is the exact index.
+__device__ __forceinline__ void axis_bd(const double* src, unsigned stride, int q, int n,
+                                        unsigned uid, double ihc, double m, double bd[6]) {
+  double s[7];
 #pragma unroll
-  for (int o = 0; o < 6; o++) {
-    int t = q - 2 + o;
+  for (int o = 0; o < 7; o++) {
+    int t = q - 3 + o;
     t = t < 0 ? 0 : (t > n - 1 ? n - 1 : t);
-    const long long it = id + (long long)(t - q) * stride;
-    bd[o] = (t == 0 ? ihc * (src[it + stride] - src[it]) : ihc * (src[it] - src[it - stride])) * m;
+    s[o] = src[uid + (unsigned)(t - q) * stride];
+  }
+#pragma unroll
+  for (int o = 0; o < 6; o++) bd[o] = ihc * (s[o + 1] - s[o]) * m;
+  if (q == 0) bd[2] = bd[3];
+  if (q == 1) bd[1] = bd[2];
+  if (q == 2) bd[0] = bd[1];
+}
+
+// The five WENO5 operands of the left-biased side along axis c from bd
+// (PhiDevPlusParameterJB, lsTools.cu:587-673: positions q-2..q+2), with the
+// reference's substitutions at the edges.  On the z axis the last plane's
+// fifth operand is the z difference one row down (id_kp2 = id_jm1,
+// lsTools.cu:636).
+__device__ __forceinline__ void stencil_left(const double* src, PDims g, int c, int q, int n,
+                                             int x, int y, int z, double ihc, double m,
+                                             const double bd[6], double b[5]) {
+#pragma unroll
+  for (int t = 0; t < 5; t++) b[t] = bd[t];
+  if (q == 0) { b[1] = bd[2]; b[0] = bd[3]; }
+  if (q == 1) b[0] = bd[1];
+  if (q == n - 2) b[4] = bd[3];
+  if (q == n - 1) {
+    b[3] = bd[2];
+    if (c < 2) b[4] = bd[1];
+    else b[4] = bdiff(src, g, 2, x, y == 0 ? y : y - 1, z, ihc) * m;
   }
 }
 
-// The five WENO5 operands of one side along axis c from bd, with the
-// reference's substitutions at the edges: left = PhiDevPlusParameterJB
-// (lsTools.cu:587-673, positions q-2..q+2), right = PhiDevMinusParameterJB
-// (:676-770, positions q+3, q+2, q+1, q, q-1).
-__device__ __forceinline__ void axis_stencil(const double* src, PDims g, int c, int q, int n,
-                                             int x, int y, int z, double ihc, double m,
-                                             const double bd[6], bool left, double b[5]) {
-  if (left) {
-    int l[5] = {q - 2, q - 1, q, q + 1, q + 2};
-    bool zquirk = false;
-    if (q == 0) { l[1] = q; l[0] = q + 1; }
-    if (q == 1) { l[0] = l[1]; }
-    if (c < 2) {
-      if (q == n - 1) { l[3] = q; l[4] = l[1]; }
-      if (q == n - 2) { l[4] = l[3]; }
-    } else {
-      if (q == n - 1) { l[3] = q; zquirk = true; }  // id_kp2 = id_jm1 (lsTools.cu:636)
-      if (q == n - 2) { l[4] = l[3]; }
-    }
+// The right-biased side (PhiDevMinusParameterJB, lsTools.cu:676-770:
+// positions q+3, q+2, q+1, q, q-1) with its edge substitutions.
+__device__ __forceinline__ void stencil_right(int q, int n, const double bd[6], double b[5]) {
 #pragma unroll
-    for (int t = 0; t < 5; t++) b[t] = bd[l[t] - q + 2];
-    if (zquirk) {
-      const int jm1y = y == 0 ? y : y - 1;  // where id_jm1 ends up
-      b[4] = bdiff(src, g, 2, x, jm1y, z, ihc) * m;
-    }
-  } else {
-    int rr[5] = {q + 3, q + 2, q + 1, q, q - 1};
-    if (q == 0) { rr[4] = q; }
-    if (q == n - 1) { rr[2] = q; rr[1] = rr[4]; rr[0] = q - 2; }
-    if (q == n - 2) { rr[1] = rr[2]; rr[0] = q; }
-    if (q == n - 3) { rr[0] = rr[1]; }
-#pragma unroll
-    for (int t = 0; t < 5; t++) b[t] = bd[rr[t] - q + 2];
-  }
+  for (int t = 0; t < 5; t++) b[t] = bd[5 - t];
+  if (q == 0) b[4] = bd[2];
+  if (q == n - 1) { b[2] = bd[2]; b[1] = bd[1]; b[0] = bd[0]; }
+  if (q == n - 2) { b[1] = bd[3]; b[0] = bd[2]; }
+  if (q == n - 3) b[0] = bd[4];
 }
 
 // One fused SSP-RK3 stage of cuAdvectLsJB (lsTransportEc.cu:243-399):
@@ -353,9 +356,11 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
       continue;
     }
     const int pos[3] = {x, y, z}, ext[3] = {g.x, g.y, g.z};
+    const unsigned stride[3] = {1u, (unsigned)g.x, (unsigned)g.x * (unsigned)g.y};
     double bd[6], b[5];
-    axis_bd(src, g, c, pos[c], ext[c], id, ih[c], met[c], bd);
-    axis_stencil(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, v > 0.0, b);
+    axis_bd(src, stride[c], pos[c], ext[c], uid, ih[c], met[c], bd);
+    if (v > 0.0) stencil_left(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, b);
+    else stencil_right(pos[c], ext[c], bd, b);
     r[c] = v * weno5(b[0], b[1], b[2], b[3], b[4]);
   }
   const double rs = r[0] + r[1] + r[2];
@@ -423,11 +428,12 @@ __global__ void k_ls_reinit_stage(int stage, const double* src, const double* P,
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const int pos[3] = {x, y, z}, ext[3] = {g.x, g.y, g.z};
+      const unsigned stride[3] = {1u, (unsigned)g.x, (unsigned)g.x * (unsigned)g.y};
       double bd[6], b[5];
-      axis_bd(src, g, c, pos[c], ext[c], id, ih[c], met[c], bd);
-      axis_stencil(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, true, b);
+      axis_bd(src, stride[c], pos[c], ext[c], uid, ih[c], met[c], bd);
+      stencil_left(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, b);
       const double L = weno5(b[0], b[1], b[2], b[3], b[4]);
-      axis_stencil(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, false, b);
+      stencil_right(pos[c], ext[c], bd, b);
       const double R = weno5(b[0], b[1], b[2], b[3], b[4]);
       const double lp = L > 0.0 ? L : 0.0, rm = R < 0.0 ? R : 0.0;
       double hi = lp * lp, lo = rm * rm;
