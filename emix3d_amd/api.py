@@ -519,6 +519,8 @@ class Batch:
                   _ptr(div), *map(float, ih), *map(float, dxyz), float(dt))
 
     def set_plan(self, direction, lines_per_wave, chunk):
+        """Line-solver shape of one direction: lines_per_wave 16/32/64 (the
+        strided kind) or 0 (one wavefront per line, direction 0 only)."""
         _lib.call("emx_batch_set_plan", self._b, direction, lines_per_wave, chunk)
 
     def get_plan(self, direction):

@@ -26,11 +26,22 @@ struct BatchShape {
   int lg, C;
 };
 
-// Shapes the batched line pass is built for (NV = 4 fields per point).
-static const BatchShape kBatchShapes[] = {{16, 4}, {16, 8}, {32, 4}, {32, 8}, {64, 8}, {64, 16}};
+// Shapes the batched line pass is built for (NV = 4 fields per point):
+// lg lines per wavefront of the strided kind, or lg = 0 for the wavefront
+// kind (one wavefront per line, direction 0 only: its lines are contiguous,
+// so each load instruction of a wave reads consecutive points), where each
+// box takes the shortest of the chunk lengths 1, 2, 4, 8 up to C that fits.
+static const BatchShape kBatchShapes[] = {{16, 4}, {16, 8}, {32, 4}, {32, 8}, {64, 8},
+                                          {64, 16}, {0, 1}, {0, 2}, {0, 4}, {0, 8}};
+constexpr int kWaveLinesPerBlock = 16;  // four per wave
 
 static int lg_kind(int lg) {
-  return lg == 64 ? KIND_STRIDED : (lg == 32 ? KIND_STRIDED32 : KIND_STRIDED16);
+  return lg == 0 ? KIND_WAVE
+                 : (lg == 64 ? KIND_STRIDED : (lg == 32 ? KIND_STRIDED32 : KIND_STRIDED16));
+}
+
+static size_t wave_lds(int C) {
+  return (size_t)(5 * 64 * C + 768 + 4 * 64 * (C + 1)) * sizeof(double);
 }
 
 template <int S, int C, int LG>
@@ -39,7 +50,18 @@ static void* batched_kernel() {
 }
 
 template <int S>
+static void* batched_wave_kernel(int C) {
+  switch (C) {
+    case 1: return (void*)k_lines_batched_wave<S, 1, ProbGradDiv>;
+    case 2: return (void*)k_lines_batched_wave<S, 2, ProbGradDiv>;
+    case 4: return (void*)k_lines_batched_wave<S, 4, ProbGradDiv>;
+    default: return (void*)k_lines_batched_wave<S, 8, ProbGradDiv>;
+  }
+}
+
+template <int S>
 static void* batched_kernel_s(int lg, int C) {
+  if (lg == 0) return batched_wave_kernel<S>(C);
   if (lg == 16) return C == 4 ? batched_kernel<S, 4, 16>() : batched_kernel<S, 8, 16>();
   if (lg == 32) return C == 4 ? batched_kernel<S, 4, 32>() : batched_kernel<S, 8, 32>();
   return C == 8 ? batched_kernel<S, 8, 64>() : batched_kernel<S, 16, 64>();
@@ -83,6 +105,8 @@ struct emx_batch {
   emx::BatchShape shape[3];
   std::vector<emx::BatchLines<emx::ProbGradDiv>> lines_h[3];
   emx::BatchLines<emx::ProbGradDiv>* lines_d[3] = {nullptr, nullptr, nullptr};
+  std::vector<emx::BatchWaveLines<emx::ProbGradDiv>> waves_h;  // direction 0
+  emx::BatchWaveLines<emx::ProbGradDiv>* waves_d = nullptr;
   int* blk_d[3] = {nullptr, nullptr, nullptr};
   int nblk[3] = {0, 0, 0};
   int threads[3] = {0, 0, 0};
@@ -119,6 +143,7 @@ void free_batch(emx_batch* b) {
     (void)hipFree(b->lines_d[d]);
     (void)hipFree(b->blk_d[d]);
   }
+  (void)hipFree(b->waves_d);
   (void)hipFree(b->ls_d);
   (void)hipFree(b->blk_pad_d);
   (void)hipFree(b->blk_int_d);
@@ -132,11 +157,51 @@ void free_batch(emx_batch* b) {
 int box_n(const Dims& d, int dir) { return dir == 0 ? d.nx : (dir == 1 ? d.ny : d.nz); }
 
 bool shape_fits(const emx_batch* b, int dir, BatchShape s) {
+  if (s.lg == 0 && dir != 0) return false;
   for (const Dims& d : b->dims) {
     const int n = box_n(d, dir);
-    if ((n + s.C - 1) / s.C > 1024 / s.lg) return false;
+    if ((n + s.C - 1) / s.C > (s.lg == 0 ? 64 : 1024 / s.lg)) return false;
   }
   return true;
+}
+
+// Direction 0 on the wavefront kind: kWaveLinesPerBlock lines per workgroup,
+// each box with its shortest chunk (LDS sized for the longest, s.C).
+int plan_wave_lines(emx_batch* b, BatchShape s) {
+  auto& H = b->waves_h;
+  H.assign(b->nbox, BatchWaveLines<ProbGradDiv>{});
+  std::vector<int> blk(b->nbox + 1, 0);
+  for (int i = 0; i < b->nbox; i++) {
+    const Dims& d = b->dims[i];
+    const DirGeom g = dir_geom(d, 0);
+    BatchWaveLines<ProbGradDiv>& L = H[i];
+    int C = 1;
+    while (C < s.C && g.n > 64 * C) C *= 2;
+    EMX_TRY(get_line_tab(b->scheme, g.n, C, KIND_WAVE, &L.T));
+    L.nlines = g.nlines;
+    L.nyl = g.nyl;
+    L.sj = g.sj;
+    L.sk = g.sk;
+    L.sl = g.sl;
+    L.nblk = (int)((g.nlines + kWaveLinesPerBlock - 1) / kWaveLinesPerBlock);
+    const emx_batch::Box& B = b->box[i];
+    L.ih = B.ih[0];
+    L.prob = ProbGradDiv{B.u, B.jbn, B.dvel, b->acc + b->acc_off[i], B.div, d.N(), 0};
+    blk[i + 1] = blk[i] + L.nblk;
+  }
+  b->shape[0] = s;
+  b->nblk[0] = blk[b->nbox];
+  b->threads[0] = 256;
+  b->lds[0] = wave_lds(s.C);
+  void* k = batched_kernel_for(b->scheme, 0, s.C);
+  if (b->lds[0] > 65536)
+    EMX_TRY_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)b->lds[0]));
+  EMX_TRY_HIP(hipMemcpy(b->waves_d, H.data(), H.size() * sizeof(H[0]), hipMemcpyHostToDevice));
+  EMX_TRY_HIP(hipMemcpy(b->blk_d[0], blk.data(), blk.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
+  drop_graph(b);
+  return EMX_OK;
 }
 
 // Tables, descriptors and launch geometry of direction dir under shape s.
@@ -146,6 +211,7 @@ int plan_lines(emx_batch* b, int dir, BatchShape s) {
               dir, s.lg, s.C);
     return EMX_ERR_SIZE;
   }
+  if (s.lg == 0) return plan_wave_lines(b, s);
   auto& H = b->lines_h[dir];
   H.assign(b->nbox, BatchLines<ProbGradDiv>{});
   std::vector<int> blk(b->nbox + 1, 0);
@@ -187,7 +253,7 @@ int plan_lines(emx_batch* b, int dir, BatchShape s) {
 
 int launch_pass(emx_batch* b, int dir) {
   void* k = batched_kernel_for(b->scheme, b->shape[dir].lg, b->shape[dir].C);
-  const BatchLines<ProbGradDiv>* grids = b->lines_d[dir];
+  const void* grids = b->shape[dir].lg == 0 ? (const void*)b->waves_d : (const void*)b->lines_d[dir];
   const int* blk = b->blk_d[dir];
   int ngrid = b->nbox;
   void* args[] = {(void*)&grids, (void*)&blk, (void*)&ngrid};
@@ -302,7 +368,9 @@ int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme, voi
             hipMalloc(&b->ls_d, nbox * sizeof(emx::LsBatchGrid)) == hipSuccess &&
             hipMalloc(&b->blk_pad_d, (nbox + 1) * sizeof(int)) == hipSuccess &&
             hipMalloc(&b->blk_int_d, (nbox + 1) * sizeof(int)) == hipSuccess &&
-            hipMalloc(&b->jbn_d, nbox * sizeof(double*)) == hipSuccess;
+            hipMalloc(&b->jbn_d, nbox * sizeof(double*)) == hipSuccess &&
+            hipMalloc(&b->waves_d, nbox * sizeof(emx::BatchWaveLines<emx::ProbGradDiv>)) ==
+                hipSuccess;
   for (int d = 0; d < 3 && ok; d++)
     ok = hipMalloc(&b->lines_d[d], nbox * sizeof(emx::BatchLines<emx::ProbGradDiv>)) ==
              hipSuccess &&
@@ -313,11 +381,14 @@ int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme, voi
     emx::free_batch(b);
     return EMX_ERR_NOMEM;
   }
-  // default shape: 16 lines per wavefront, the shortest chunk that fits
+  // default shape: 16 lines per wavefront, the shortest chunk that fits;
+  // direction 0 on the wavefront kind with the shortest chunk that fits
   for (int d = 0; d < 3; d++) {
     b->shape[d] = emx::BatchShape{16, 4};
     if (!emx::shape_fits(b, d, b->shape[d])) b->shape[d] = emx::BatchShape{16, 8};
   }
+  for (int C : {8, 4, 2, 1})
+    if (emx::shape_fits(b, 0, emx::BatchShape{0, C})) b->shape[0] = emx::BatchShape{0, C};
   *out = b;
   return EMX_OK;
 }

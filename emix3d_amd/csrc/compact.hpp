@@ -540,6 +540,45 @@ k_lines_batched(const BatchLines<Prob>* grids, const int* blk_start, int ngrid) 
                                line_group<LG>(local % G.gx, G.gx), local / G.gx);
 }
 
+// The wavefront kind for many grids in one launch (xi lines, contiguous in
+// memory): workgroup b belongs to grid g as above and is workgroup
+// b - blk_start[g] of the nblk that grid's lines are dealt to (its four
+// waves take every (4*nblk)-th line).  Each grid has its own chunk length
+// (its table's C, at most CMAX): the branch is uniform over a workgroup.
+template <class Prob>
+struct BatchWaveLines {
+  Prob prob;
+  LineTab T;
+  long long nlines, sj, sk, sl;
+  int nyl, nblk;
+  double ih;
+};
+
+template <int S, int CMAX, class Prob>
+__global__ void __launch_bounds__(256)
+k_lines_batched_wave(const BatchWaveLines<Prob>* grids, const int* blk_start, int ngrid) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = ngrid;  // blk_start[lo] <= b < blk_start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (blk_start[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  const BatchWaveLines<Prob>& G = grids[lo];
+  const int blk = b - blk_start[lo];
+  if (CMAX >= 8 && G.T.C == 8)
+    lines_wave_body<S, (CMAX >= 8 ? 8 : 1)>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
+                                             G.ih, blk, G.nblk);
+  else if (CMAX >= 4 && G.T.C == 4)
+    lines_wave_body<S, (CMAX >= 4 ? 4 : 1)>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
+                                             G.ih, blk, G.nblk);
+  else if (CMAX >= 2 && G.T.C == 2)
+    lines_wave_body<S, (CMAX >= 2 ? 2 : 1)>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
+                                             G.ih, blk, G.nblk);
+  else
+    lines_wave_body<S, 1>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl, G.ih, blk, G.nblk);
+}
+
 // ---------------------------------------------------------------------------
 // The strided shape for problems whose point values are expensive to form
 // (many input fields, e.g. the flux divergence): each work-item loads and

@@ -205,7 +205,10 @@ int emx_batch_bind(emx_batch* b, int i, const double* u, const double* jbn, doub
                    double dx, double dy, double dz, double dt);
 /* Line-solver shape of direction dir: lg lines per wavefront (16, 32, 64)
  * and C points per work-item; built pairs (16,4) (16,8) (32,4) (32,8)
- * (64,8) (64,16). */
+ * (64,8) (64,16).  lg = 0 (direction 0 only) is one wavefront per line,
+ * each box with the shortest of 1, 2, 4, 8 points per lane up to C that
+ * fits its lines (C = 1, 2, 4, 8; every line at most 64*C points): the
+ * default for direction 0. */
 int emx_batch_set_plan(emx_batch* b, int dir, int lg, int C);
 int emx_batch_get_plan(const emx_batch* b, int dir, int* lg, int* C);
 /* Autotuner: times every built shape of each direction's line pass on the

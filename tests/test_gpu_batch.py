@@ -16,11 +16,13 @@ import torch
 
 from conftest import normwise_rel
 from emix3d_amd import Batch, Operator, fields
+from emix3d_amd._lib import EmxError
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 DIMS = [(6, 7, 5), (24, 9, 13), (17, 20, 8), (9, 9, 30), (33, 12, 11)]
 SHAPES = [(16, 4), (16, 8), (32, 4), (32, 8), (64, 8), (64, 16)]
+WAVE_SHAPES = [(0, 1), (0, 2), (0, 4), (0, 8)]  # one wavefront per line, direction 0
 
 
 def host(t):
@@ -81,6 +83,24 @@ def test_batch_stage_vs_oracle_every_shape(oracle):
             reset(boxes)
             b.stage(graph=graph)
             check(oracle, boxes, refs, f"lg{lg} C{c} graph={graph}")
+    for lg, c in WAVE_SHAPES:
+        b.set_plan(0, lg, c)
+        b.set_plan(1, 16, 8)
+        b.set_plan(2, 32, 4)
+        for graph in (False, True):
+            reset(boxes)
+            b.stage(graph=graph)
+            check(oracle, boxes, refs, f"wave C{c} graph={graph}")
+
+
+def test_batch_wave_shape_only_along_x():
+    b, _ = make_batch(DIMS[:2])
+    assert b.get_plan(0) == (0, 1)  # default: the shortest chunk that fits
+    for d in (1, 2):
+        with pytest.raises(EmxError):
+            b.set_plan(d, 0, 4)
+    with pytest.raises(EmxError):
+        b.set_plan(0, 0, 3)  # not built
 
 
 def test_graph_replays_match_eager(oracle):
@@ -105,7 +125,7 @@ def test_autotuner_keeps_results(oracle):
     refs = [reference(oracle, z) for z in boxes]
     b.tune(reps=3)
     for d in range(3):
-        assert b.get_plan(d) in SHAPES
+        assert b.get_plan(d) in SHAPES + (WAVE_SHAPES if d == 0 else [])
     reset(boxes)
     b.stage(graph=True)
     check(oracle, boxes, refs, "tuned")
