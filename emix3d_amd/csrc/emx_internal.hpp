@@ -37,6 +37,16 @@ void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
     if (rc_ != EMX_OK) return rc_;    \
   } while (0)
 
+// Workgroups go to the eight XCDs round robin (XCD = id mod 8), each XCD
+// with its own L2.  Renumbered by xcd_block, each XCD works through one
+// contiguous eighth of the block index space, so the stencil neighbours of
+// its points (rows and planes away, or the neighbouring tiles) sit in the
+// same L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned q = nb / 8;
+  return b < 8 * q ? (b % 8) * q + b / 8 : b;
+}
+
 // Kinds of line solver launch (see compact.hpp).
 enum LineKind {
   KIND_WAVE = 0,       // one wavefront per line, lanes along the line

@@ -27,15 +27,6 @@ __device__ __forceinline__ int mirror(int i, int n) {
   return s - GC;
 }
 
-// Workgroups go to the eight XCDs round robin (XCD = id mod 8), each XCD
-// with its own L2.  Renumbered by xcd_block, each XCD works through one
-// contiguous eighth of the index space, so the stencil neighbours of its
-// points (rows and planes away) sit in the same L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
-  const unsigned q = nb / 8;
-  return b < 8 * q ? (b % 8) * q + b / 8 : b;
-}
-
 // cuSwapToGhost + the three mirror passes, as one gather.
 __global__ void k_pad(const double* in, int nx, int ny, int nz, double* out,
                       PDims g) {

@@ -69,7 +69,10 @@ int op_sgs_geometry(emx_ctx* ctx, const double* x, double* sgd,
 // through a slab; the velocity and the vorticity of planes k-1, k, k+1 are
 // held in LDS with a one-point halo in i and j (rolling over three slots),
 // so every input is read from memory once per workgroup instead of once
-// per neighbour, and the vorticity never goes to memory.
+// per neighbour, and the vorticity never goes to memory.  Tiles are
+// numbered i fastest, then j, then slab, and dealt to the XCDs in
+// contiguous ranges (xcd_block): the tiles that share a halo run on the
+// same XCD at the same time, so the halo's cache lines come from its L2.
 constexpr int SG_TX = 32, SG_TY = 8, SG_HX = SG_TX + 2, SG_HY = SG_TY + 2,
               SG_HP = SG_HX * SG_HY;
 
@@ -81,8 +84,10 @@ k_sgs_fused(const double* u, const double* dvel, const double* jbn, const double
   const long long N = d.N();
   const long long sy = d.nx, sz = (long long)d.nx * d.ny;
   const int tx = threadIdx.x % SG_TX, ty = threadIdx.x / SG_TX;
-  const int i0 = blockIdx.x * SG_TX, j0 = blockIdx.y * SG_TY;
-  const int k0 = blockIdx.z * kz, k1 = min(k0 + kz, d.nz);
+  const unsigned nbx = (d.nx + SG_TX - 1) / SG_TX, nby = (d.ny + SG_TY - 1) / SG_TY;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (int)(b % nbx) * SG_TX, j0 = (int)((b / nbx) % nby) * SG_TY;
+  const int k0 = (int)(b / (nbx * nby)) * kz, k1 = min(k0 + kz, d.nz);
   const int i = i0 + tx, j = j0 + ty;
   const bool own = i < d.nx && j < d.ny;
   const double amu0 = 0.1781;
@@ -190,7 +195,9 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
             double* vis, double* vis5) {
   const Dims d = ctx->d;
   const int kz = 16;
-  dim3 grid((d.nx + SG_TX - 1) / SG_TX, (d.ny + SG_TY - 1) / SG_TY, (d.nz + kz - 1) / kz);
+  const unsigned tiles = (unsigned)((d.nx + SG_TX - 1) / SG_TX) *
+                         (unsigned)((d.ny + SG_TY - 1) / SG_TY) * (unsigned)((d.nz + kz - 1) / kz);
+  const dim3 grid(tiles);
   // reads u (3), dvel (6), jbn (3), sgd (7); writes amut, vis (, vis5)
   EMX_TIMED(ctx, "sgdm", (vis5 ? 176.0 : 168.0) * d.N(),
             hipLaunchKernelGGL(k_sgs_fused, grid, dim3(256), 0, ctx->stream, u, dvel, jbn,
