@@ -7,6 +7,8 @@ per phase over all boxes, replayed from a HIP graph.
     eager launch, under every built line-solver shape;
   * graph replays, with eager work and synchronisations between them, equal
     the eager result each time;
+  * the wavefront kind along x with each box on its own chunk length (1, 2,
+    4 and 8 points per lane in one launch), and its shape checks;
   * the autotuner keeps a built shape and the results do not change;
   * the single-grid entry emx_grad_div gives the same fields.
 """
@@ -91,6 +93,23 @@ def test_batch_stage_vs_oracle_every_shape(oracle):
             reset(boxes)
             b.stage(graph=graph)
             check(oracle, boxes, refs, f"wave C{c} graph={graph}")
+
+
+def test_batch_wave_chunk_per_box(oracle):
+    # x lines of 20, 70, 130 and 260 points: 1, 2, 4 and 8 points per lane in
+    # one launch of the wavefront kind (each box its shortest chunk)
+    dims = [(70, 9, 8), (260, 6, 5), (20, 8, 7), (130, 7, 6)]
+    b, boxes = make_batch(dims)
+    refs = [reference(oracle, z) for z in boxes]
+    assert b.get_plan(0) == (0, 8)  # the only wavefront shape that fits 260
+    with pytest.raises(EmxError):
+        b.set_plan(0, 0, 4)
+    b.set_plan(1, 16, 4)
+    b.set_plan(2, 16, 4)
+    for graph in (False, True):
+        reset(boxes)
+        b.stage(graph=graph)
+        check(oracle, boxes, refs, f"wave per-box chunks graph={graph}")
 
 
 def test_batch_wave_shape_only_along_x():
