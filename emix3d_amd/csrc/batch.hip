@@ -99,7 +99,7 @@ struct emx_batch {
   std::vector<Box> box;
   // scratch
   double* acc = nullptr;        // sum of N
-  double* ls = nullptr;         // sum of 9 * NG
+  double* ls = nullptr;         // sum of 6 * NG
   std::vector<long long> acc_off, ls_off;
   // line passes, per direction
   emx::BatchShape shape[3];
@@ -291,10 +291,9 @@ int commit(emx_batch* b) {
     G.phi = B.phi;
     G.vel = B.u;
     G.P = base;
-    G.V = base + NG;
-    G.J = base + 4 * NG;
-    G.T1 = base + 7 * NG;
-    G.T2 = base + 8 * NG;
+    G.J = base + NG;
+    G.T1 = base + 4 * NG;
+    G.T2 = base + 5 * NG;
     G.iX = 1.0 / B.dxyz[0];
     G.iY = 1.0 / B.dxyz[1];
     G.iZ = 1.0 / B.dxyz[2];
@@ -352,7 +351,7 @@ int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme, voi
     b->acc_off.push_back(accN);
     b->ls_off.push_back(lsN);
     accN += d.N();
-    lsN += 9 * emx::ls_padded_points(d.nx, d.ny, d.nz);
+    lsN += 6 * emx::ls_padded_points(d.nx, d.ny, d.nz);
   }
   b->box.resize(nbox);
   if (hipGetDevice(&b->device) != hipSuccess ||

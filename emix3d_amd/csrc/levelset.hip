@@ -321,7 +321,10 @@ __device__ __forceinline__ void stencil_right(int q, int n, const double bd[6], 
 // One fused SSP-RK3 stage of cuAdvectLsJB (lsTransportEc.cu:243-399):
 // backward differences, the upwind JB WENO derivative, the right-hand side
 // advect_RS_WENO (:361-417) and the stage update.  src is the stage input,
-// P the step's initial phi.
+// P the step's initial phi.  vel is the interior velocity (3 components of
+// (nx, ny, nz)): the padded velocity the reference builds is its mirror
+// image, so a padded point reads the interior point at its mirrored
+// coordinates instead of a padded copy.
 __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                                                  const double* P, const double* vel,
                                                  const double* j1, const double* j5,
@@ -334,6 +337,11 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
             z = (int)(row / (unsigned)g.y);
   const double ih[3] = {iX, iY, iZ};
   const double met[3] = {j1[id], j5[id], j9[id]};
+  const unsigned nx = (unsigned)(g.x - 2 * GC), ny = (unsigned)(g.y - 2 * GC),
+                 nz = (unsigned)(g.z - 2 * GC);
+  const unsigned vid = (unsigned)mirror(x, (int)nx) +
+                       nx * ((unsigned)mirror(y, (int)ny) + ny * (unsigned)mirror(z, (int)nz));
+  const unsigned vN = nx * ny * nz;
   double r[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -341,7 +349,7 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
     // derivative where the velocity is positive and the right-biased one
     // where it is negative, so only that one is formed:
     //   0.5(s+1) v L + 0.5|s-1| v R  =  v L (v > 0),  v R (v < 0),  0
-    const double v = vel[c * NG + id];
+    const double v = vel[(size_t)c * vN + vid];
     if (v == 0.0) {
       r[c] = 0.0;
       continue;
@@ -471,12 +479,9 @@ k_batch_pad(const LsBatchGrid* grids, const int* blk_start, int ngrid) {
   const unsigned up = (unsigned)p, row = up / (unsigned)g.x;
   const int i = (int)(up - row * (unsigned)g.x), j = (int)(row % (unsigned)g.y),
             k = (int)(row / (unsigned)g.y);
-  const long long N = (long long)G.nx * G.ny * G.nz, NG = g.N();
   const long long src = mirror(i, G.nx) + (long long)G.nx * (mirror(j, G.ny) +
                                                              (long long)G.ny * mirror(k, G.nz));
   G.P[p] = G.phi[src];
-#pragma unroll
-  for (int c = 0; c < 3; c++) G.V[c * NG + p] = G.vel[c * N + src];
 }
 
 __global__ void __launch_bounds__(256)
@@ -489,7 +494,7 @@ k_batch_weno_stage(const LsBatchGrid* grids, const int* blk_start, int ngrid, in
   if (id >= NG) return;
   const double* src = stage == 0 ? G.P : (stage == 1 ? G.T1 : G.T2);
   double* out = stage == 0 ? G.T1 : (stage == 1 ? G.T2 : G.P);
-  weno_stage_point(stage, src, G.P, G.V, G.J, G.J + NG, G.J + 2 * NG, g, G.iX, G.iY, G.iZ,
+  weno_stage_point(stage, src, G.P, G.vel, G.J, G.J + NG, G.J + 2 * NG, g, G.iX, G.iY, G.iZ,
                    G.dt, out, id);
 }
 
@@ -702,8 +707,7 @@ int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
   const PDims g = padded(d);
   const long long N = d.N(), NG = g.N();
   double* P = ctx->ls_buf;
-  double* V = P + NG;  // 3*NG
-  double* J1 = V + 3 * NG;
+  double* J1 = P + NG;
   double* J5 = J1 + NG;
   double* J9 = J5 + NG;
   double* T1 = J9 + NG;
@@ -712,10 +716,6 @@ int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
   const unsigned bg = blocks_for(NG);
   EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
           hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, phi, d.nx, d.ny, d.nz, P, g));
-  for (int c = 0; c < 3; c++)
-    EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
-          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, vel + c * N, d.nx, d.ny,
-                       d.nz, V + c * NG, g));
   EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
           hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g));
   EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
@@ -723,14 +723,16 @@ int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
   EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
           hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g));
   const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz;
-  EMX_TIMED(ctx, "ls_weno_stage", 64.0 * NG,
-          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, V, J1,
+  // per stage: phi in (and the step's phi), the metrics, phi out on the
+  // padded grid; the velocity once on the interior grid
+  EMX_TIMED(ctx, "ls_weno_stage", 40.0 * NG + 24.0 * N,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, vel, J1,
                      J5, J9, g, iX, iY, iZ, dt, T1));
-  EMX_TIMED(ctx, "ls_weno_stage", 72.0 * NG,
-          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, V, J1,
+  EMX_TIMED(ctx, "ls_weno_stage", 48.0 * NG + 24.0 * N,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, vel, J1,
                      J5, J9, g, iX, iY, iZ, dt, T2));
-  EMX_TIMED(ctx, "ls_weno_stage", 72.0 * NG,
-          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, V, J1,
+  EMX_TIMED(ctx, "ls_weno_stage", 48.0 * NG + 24.0 * N,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, vel, J1,
                      J5, J9, g, iX, iY, iZ, dt, P));
   EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
           hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,

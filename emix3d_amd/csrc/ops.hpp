@@ -77,7 +77,6 @@ struct LsBatchGrid {
   double* phi;
   const double* vel;
   double* P;
-  double* V;
   double* J;
   double* T1;
   double* T2;
