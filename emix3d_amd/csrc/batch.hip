@@ -1,10 +1,11 @@
 // Batched multi-grid stage: many independent grids ("boxes") of uneven
 // sizes advanced together.  One stage = the velocity gradient fused with
 // the velocity divergence (ProbGradDiv, three directional line passes) and
-// one WENO5 / SSP-RK3 level-set step (pad, three stages, unpad).  Every
+// one WENO5 / SSP-RK3 level-set step (pad, three stages, the last writing
+// the interior straight into each box's phi).  Every
 // phase is ONE launch over all boxes: workgroups are dealt out box by box
 // from per-phase prefix tables (blk_start), each box with its own line
-// tables, geometry and data.  The phase sequence (eight launches) is
+// tables, geometry and data.  The phase sequence (seven launches) is
 // captured once in a HIP graph and replayed.  The line-solver shape of
 // each direction (lines per wavefront LG and points per work-item C, i.e.
 // the split between the serial Thomas sweep inside a chunk and the
@@ -115,9 +116,8 @@ struct emx_batch {
   std::vector<emx::LsBatchGrid> ls_h;
   emx::LsBatchGrid* ls_d = nullptr;
   int* blk_pad_d = nullptr;
-  int* blk_int_d = nullptr;
   const double** jbn_d = nullptr;
-  int nblk_pad = 0, nblk_int = 0;
+  int nblk_pad = 0;
   // state
   bool dirty = true;
   hipGraph_t graph = nullptr;
@@ -146,7 +146,6 @@ void free_batch(emx_batch* b) {
   (void)hipFree(b->waves_d);
   (void)hipFree(b->ls_d);
   (void)hipFree(b->blk_pad_d);
-  (void)hipFree(b->blk_int_d);
   (void)hipFree(b->jbn_d);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_out) (void)hipEventDestroy(b->ev_out);
@@ -264,8 +263,7 @@ int launch_pass(emx_batch* b, int dir) {
 
 int launch_stage(emx_batch* b) {
   for (int d = 0; d < 3; d++) EMX_TRY(launch_pass(b, d));
-  return op_batch_ls_step(b->stream, b->ls_d, b->blk_pad_d, b->nblk_pad, b->blk_int_d,
-                          b->nblk_int, b->nbox);
+  return op_batch_ls_step(b->stream, b->ls_d, b->blk_pad_d, b->nblk_pad, b->nbox);
 }
 
 // Descriptors of every phase from the bound boxes; pads the metrics.
@@ -276,7 +274,7 @@ int commit(emx_batch* b) {
       return EMX_ERR_ARG;
     }
   for (int d = 0; d < 3; d++) EMX_TRY(plan_lines(b, d, b->shape[d]));
-  std::vector<int> bp(b->nbox + 1, 0), bi(b->nbox + 1, 0);
+  std::vector<int> bp(b->nbox + 1, 0);
   std::vector<const double*> jb(b->nbox);
   b->ls_h.resize(b->nbox);
   for (int i = 0; i < b->nbox; i++) {
@@ -299,15 +297,12 @@ int commit(emx_batch* b) {
     G.iZ = 1.0 / B.dxyz[2];
     G.dt = B.dt;
     bp[i + 1] = bp[i] + (int)((NG + 255) / 256);
-    bi[i + 1] = bi[i] + (int)((d.N() + 255) / 256);
     jb[i] = B.jbn;
   }
   b->nblk_pad = bp[b->nbox];
-  b->nblk_int = bi[b->nbox];
   EMX_TRY_HIP(hipMemcpy(b->ls_d, b->ls_h.data(), b->nbox * sizeof(LsBatchGrid),
                         hipMemcpyHostToDevice));
   EMX_TRY_HIP(hipMemcpy(b->blk_pad_d, bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
-  EMX_TRY_HIP(hipMemcpy(b->blk_int_d, bi.data(), bi.size() * sizeof(int), hipMemcpyHostToDevice));
   EMX_TRY_HIP(hipMemcpy(b->jbn_d, jb.data(), jb.size() * sizeof(double*), hipMemcpyHostToDevice));
   EMX_TRY(op_batch_ls_pad_metrics(b->stream, b->ls_d, b->blk_pad_d, b->nblk_pad, b->nbox,
                                   b->jbn_d));
@@ -366,7 +361,6 @@ int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme, voi
             hipMalloc(&b->ls, lsN * sizeof(double)) == hipSuccess &&
             hipMalloc(&b->ls_d, nbox * sizeof(emx::LsBatchGrid)) == hipSuccess &&
             hipMalloc(&b->blk_pad_d, (nbox + 1) * sizeof(int)) == hipSuccess &&
-            hipMalloc(&b->blk_int_d, (nbox + 1) * sizeof(int)) == hipSuccess &&
             hipMalloc(&b->jbn_d, nbox * sizeof(double*)) == hipSuccess &&
             hipMalloc(&b->waves_d, nbox * sizeof(emx::BatchWaveLines<emx::ProbGradDiv>)) ==
                 hipSuccess;

@@ -321,7 +321,10 @@ __device__ __forceinline__ void stencil_right(int q, int n, const double bd[6], 
 // One fused SSP-RK3 stage of cuAdvectLsJB (lsTransportEc.cu:243-399):
 // backward differences, the upwind JB WENO derivative, the right-hand side
 // advect_RS_WENO (:361-417) and the stage update.  src is the stage input,
-// P the step's initial phi.  vel is the interior velocity (3 components of
+// P the step's initial phi; the stage writes out at every padded point, or,
+// given phi_int (the last stage), only the interior points, straight into
+// the unpadded phi (the values the reference's cuSwapFromGhost copies
+// back).  vel is the interior velocity (3 components of
 // (nx, ny, nz)): the padded velocity the reference builds is its mirror
 // image, so a padded point reads the interior point at its mirrored
 // coordinates instead of a padded copy.
@@ -330,11 +333,13 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                                                  const double* j1, const double* j5,
                                                  const double* j9, PDims g, double iX,
                                                  double iY, double iZ, double dt,
-                                                 double* out, long long id) {
-  const long long NG = g.N();
+                                                 double* out, double* phi_int, long long id) {
   const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
   const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
             z = (int)(row / (unsigned)g.y);
+  if (phi_int && (x < GC || x >= g.x - GC || y < GC || y >= g.y - GC || z < GC ||
+                  z >= g.z - GC))
+    return;  // a ghost point of the last stage: not part of the result
   const double ih[3] = {iX, iY, iZ};
   const double met[3] = {j1[id], j5[id], j9[id]};
   const unsigned nx = (unsigned)(g.x - 2 * GC), ny = (unsigned)(g.y - 2 * GC),
@@ -364,20 +369,25 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
   }
   const double rs = r[0] + r[1] + r[2];
   const double sv = src[id];
-  if (stage == 0) out[id] = sv - dt * rs;
-  else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (sv - dt * rs);
-  else out[id] = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+  double o;
+  if (stage == 0) o = sv - dt * rs;
+  else if (stage == 1) o = 0.75 * P[id] + 0.25 * (sv - dt * rs);
+  else o = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+  if (phi_int)
+    phi_int[(unsigned)(x - GC) + nx * ((unsigned)(y - GC) + ny * (unsigned)(z - GC))] = o;
+  else
+    out[id] = o;
 }
 
 __global__ void k_weno_stage(int stage, const double* src, const double* P,
                              const double* vel, const double* j1,
                              const double* j5, const double* j9, PDims g,
                              double iX, double iY, double iZ, double dt,
-                             double* out) {
+                             double* out, double* phi_int) {
   const long long NG = g.N();
   for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
        id += (long long)gridDim.x * blockDim.x)
-    weno_stage_point(stage, src, P, vel, j1, j5, j9, g, iX, iY, iZ, dt, out, id);
+    weno_stage_point(stage, src, P, vel, j1, j5, j9, g, iX, iY, iZ, dt, out, phi_int, id);
 }
 
 // ---------------------------------------------------------------------------
@@ -493,23 +503,9 @@ k_batch_weno_stage(const LsBatchGrid* grids, const int* blk_start, int ngrid, in
   const long long id = (long long)((int)xcd_block(blockIdx.x, gridDim.x) - blk_start[gi]) * 256 + threadIdx.x;
   if (id >= NG) return;
   const double* src = stage == 0 ? G.P : (stage == 1 ? G.T1 : G.T2);
-  double* out = stage == 0 ? G.T1 : (stage == 1 ? G.T2 : G.P);
+  double* out = stage == 0 ? G.T1 : G.T2;
   weno_stage_point(stage, src, G.P, G.vel, G.J, G.J + NG, G.J + 2 * NG, g, G.iX, G.iY, G.iZ,
-                   G.dt, out, id);
-}
-
-__global__ void __launch_bounds__(256)
-k_batch_unpad(const LsBatchGrid* grids, const int* blk_start, int ngrid) {
-  const int gi = batch_grid(blk_start, ngrid);
-  const LsBatchGrid& G = grids[gi];
-  const long long p = (long long)((int)xcd_block(blockIdx.x, gridDim.x) - blk_start[gi]) * 256 + threadIdx.x;
-  const long long N = (long long)G.nx * G.ny * G.nz;
-  if (p >= N) return;
-  const int gx = G.nx + 2 * GC, gy = G.ny + 2 * GC;
-  const unsigned up = (unsigned)p, row = up / (unsigned)G.nx;
-  const int i = (int)(up - row * (unsigned)G.nx), j = (int)(row % (unsigned)G.ny),
-            k = (int)(row / (unsigned)G.ny);
-  G.phi[p] = G.P[(long long)(i + GC) + (long long)gx * ((long long)(j + GC) + (long long)gy * (k + GC))];
+                   G.dt, out, stage == 2 ? G.phi : nullptr, id);
 }
 
 __global__ void __launch_bounds__(256)
@@ -563,12 +559,11 @@ int op_batch_ls_pad_metrics(hipStream_t st, const LsBatchGrid* grids, const int*
 }
 
 int op_batch_ls_step(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
-                     int nblk_pad, const int* blk_int, int nblk_int, int ngrid) {
+                     int nblk_pad, int ngrid) {
   hipLaunchKernelGGL(k_batch_pad, dim3(nblk_pad), dim3(256), 0, st, grids, blk_pad, ngrid);
   for (int stage = 0; stage < 3; stage++)
     hipLaunchKernelGGL(k_batch_weno_stage, dim3(nblk_pad), dim3(256), 0, st, grids, blk_pad,
                        ngrid, stage);
-  hipLaunchKernelGGL(k_batch_unpad, dim3(nblk_int), dim3(256), 0, st, grids, blk_int, ngrid);
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }
@@ -727,16 +722,13 @@ int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
   // padded grid; the velocity once on the interior grid
   EMX_TIMED(ctx, "ls_weno_stage", 40.0 * NG + 24.0 * N,
           hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, vel, J1,
-                     J5, J9, g, iX, iY, iZ, dt, T1));
+                     J5, J9, g, iX, iY, iZ, dt, T1, nullptr));
   EMX_TIMED(ctx, "ls_weno_stage", 48.0 * NG + 24.0 * N,
           hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, vel, J1,
-                     J5, J9, g, iX, iY, iZ, dt, T2));
-  EMX_TIMED(ctx, "ls_weno_stage", 48.0 * NG + 24.0 * N,
+                     J5, J9, g, iX, iY, iZ, dt, T2, nullptr));
+  EMX_TIMED(ctx, "ls_weno_stage", 40.0 * NG + 32.0 * N,
           hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, vel, J1,
-                     J5, J9, g, iX, iY, iZ, dt, P));
-  EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
-          hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
-                     d.nx, d.ny, d.nz, phi));
+                     J5, J9, g, iX, iY, iZ, dt, nullptr, phi));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -83,12 +83,12 @@ struct LsBatchGrid {
   double iX, iY, iZ, dt;
 };
 long long ls_padded_points(int nx, int ny, int nz);
-// blk_pad / blk_int: per-grid first workgroup (ngrid + 1 entries) over the
-// padded and the interior points, 256 points per workgroup.
+// blk_pad: per-grid first workgroup (ngrid + 1 entries) over the padded
+// points, 256 points per workgroup.
 int op_batch_ls_pad_metrics(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
                             int nblk_pad, int ngrid, const double* const* jbn);
 int op_batch_ls_step(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
-                     int nblk_pad, const int* blk_int, int nblk_int, int ngrid);
+                     int nblk_pad, int ngrid);
 
 // level set (levelset.hip); scratch is ctx->ls_buf
 size_t ls_scratch_doubles(const Dims& d);
