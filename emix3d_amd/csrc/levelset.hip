@@ -54,41 +54,35 @@ __global__ void k_unpad(const double* in, PDims g, int nx, int ny, int nz,
   }
 }
 
-// DevFirstOrder_LS (src/LevelSet/extrapol.cu:105-153).
-__global__ void k_ls_grad1(const double* S, const double* j1, const double* j5,
-                           const double* j9, PDims g, double iX, double iY,
-                           double iZ, double* dphi) {
-  const long long NG = g.N();
+// DevFirstOrder_LS (src/LevelSet/extrapol.cu:105-153) at padded point id:
+// the metric-scaled central difference of S along each axis, one-sided with
+// factor 1 for all three when the point lies on any face of the padded grid.
+__device__ __forceinline__ void ls_grad1_point(const double* S, const double m[3], PDims g,
+                                               double iX, double iY, double iZ, long long id,
+                                               int x, int y, int z, double dphi[3]) {
   const long long sxy = (long long)g.x * g.y;
-  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
-       id += (long long)gridDim.x * blockDim.x) {
-    const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
-    const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
-              z = (int)(row / (unsigned)g.y);
-    long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x,
-              kp = id + sxy, km = id - sxy;
-    double factor = 0.5;
-    if (x == 0) { im = id; factor = 1.0; }
-    if (y == 0) { jm = id; factor = 1.0; }
-    if (z == 0) { km = id; factor = 1.0; }
-    if (x == g.x - 1) { ip = id; factor = 1.0; }
-    if (y == g.y - 1) { jp = id; factor = 1.0; }
-    if (z == g.z - 1) { kp = id; factor = 1.0; }
-    dphi[id] = factor * iX * j1[id] * (S[ip] - S[im]);
-    dphi[NG + id] = factor * iY * j5[id] * (S[jp] - S[jm]);
-    dphi[2 * NG + id] = factor * iZ * j9[id] * (S[kp] - S[km]);
-  }
+  long long ip = id + 1, im = id - 1, jp = id + g.x, jm = id - g.x, kp = id + sxy,
+            km = id - sxy;
+  double factor = 0.5;
+  if (x == 0) { im = id; factor = 1.0; }
+  if (y == 0) { jm = id; factor = 1.0; }
+  if (z == 0) { km = id; factor = 1.0; }
+  if (x == g.x - 1) { ip = id; factor = 1.0; }
+  if (y == g.y - 1) { jp = id; factor = 1.0; }
+  if (z == g.z - 1) { kp = id; factor = 1.0; }
+  dphi[0] = factor * iX * m[0] * (S[ip] - S[im]);
+  dphi[1] = factor * iY * m[1] * (S[jp] - S[jm]);
+  dphi[2] = factor * iZ * m[2] * (S[kp] - S[km]);
 }
 
 // extrapolKernel (extrapol.cu:5-103, extFlag>0) at one solid point (phiS >
 // 0) of the padded grid: the upwind metric-scaled one-sided differences of
 // ext along the sign of the solid's gradient.  Elsewhere the reference's rs
-// is exactly zero (so = 0 and finite operands).
-__device__ __forceinline__ double ls_extrap_rs(long long id, const double* ext, const double* j1,
-                                               const double* j5, const double* j9,
-                                               const double* dphi, PDims g, double iX,
+// is exactly zero (so = 0 and finite operands).  geo: the point's metrics
+// 1, 5, 9 and gradient of the solid (k_ls_solid_geom).
+__device__ __forceinline__ double ls_extrap_rs(long long id, const double* ext,
+                                               const double* geo, PDims g, double iX,
                                                double iY, double iZ) {
-  const long long NG = g.N();
   const long long sxy = (long long)g.x * g.y;
   const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
   const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
@@ -103,14 +97,15 @@ __device__ __forceinline__ double ls_extrap_rs(long long id, const double* ext, 
   if (y == g.y - 1) { id2 = jm; jp = id; }
   if (z == g.z - 1) { id2 = km; kp = id; }
   const double so = 1.0;
-  const double pdx = so * dphi[id], pdy = so * dphi[NG + id], pdz = so * dphi[2 * NG + id];
+  const double m1 = geo[0], m5 = geo[1], m9 = geo[2];
+  const double pdx = so * geo[3], pdy = so * geo[4], pdz = so * geo[5];
   const double e2 = ext[id2];
-  const double xd = iX * j1[id] * (e2 - ext[im]);
-  const double yd = iY * j5[id] * (e2 - ext[jm]);
-  const double zd = iZ * j9[id] * (e2 - ext[km]);
-  const double xu = iX * j1[id] * (ext[ip] - e2);
-  const double yu = iY * j5[id] * (ext[jp] - e2);
-  const double zu = iZ * j9[id] * (ext[kp] - e2);
+  const double xd = iX * m1 * (e2 - ext[im]);
+  const double yd = iY * m5 * (e2 - ext[jm]);
+  const double zd = iZ * m9 * (e2 - ext[km]);
+  const double xu = iX * m1 * (ext[ip] - e2);
+  const double yu = iY * m5 * (ext[jp] - e2);
+  const double zu = iZ * m9 * (ext[kp] - e2);
   double s = (double)(pdx > 0.0) - (double)(pdx < 0.0);
   double r = (0.5 * (s + 1.0) * xd + 0.5 * fabs(s - 1.0) * xu) * pdx;
   s = (double)(pdy > 0.0) - (double)(pdy < 0.0);
@@ -203,14 +198,41 @@ __global__ void k_copy2(const double* a, double* b, double* c, long long n) {
 // The fluid points keep P in all three buffers: their rs is zero, and the
 // stages only re-round them (0.75*P + 0.25*P, (P + 2*P)/3), which the
 // reference does and this build does not.
+// The metrics 1, 5, 9 (read from the interior grid through the mirror map
+// of the padding) and DevFirstOrder_LS's gradient of S at each listed solid
+// point, six doubles per entry: the only points where the pseudo-stages use
+// them.
+__global__ void k_ls_solid_geom(const int* list, const int* count, const double* S,
+                                const double* jbn, int nx, int ny, int nz, PDims g, double iX,
+                                double iY, double iZ, double* geo) {
+  const int n = *count;
+  const long long N = (long long)nx * ny * nz;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const long long id = list[t];
+    const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
+    const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
+              z = (int)(row / (unsigned)g.y);
+    const long long q = mirror(x, nx) + (long long)nx * (mirror(y, ny) + (long long)ny * mirror(z, nz));
+    const double m[3] = {jbn[q], jbn[4 * N + q], jbn[8 * N + q]};
+    double dphi[3];
+    ls_grad1_point(S, m, g, iX, iY, iZ, id, x, y, z, dphi);
+    double* e = geo + 6 * (long long)t;
+    e[0] = m[0];
+    e[1] = m[1];
+    e[2] = m[2];
+    e[3] = dphi[0];
+    e[4] = dphi[1];
+    e[5] = dphi[2];
+  }
+}
+
 __global__ void k_ls_extrap_list(int stage, const int* list, const int* count, const double* ext,
-                                 const double* P, const double* j1, const double* j5,
-                                 const double* j9, const double* dphi, PDims g, double iX,
+                                 const double* P, const double* geo, PDims g, double iX,
                                  double iY, double iZ, double dt, double* out) {
   const int n = *count;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
     const long long id = list[t];
-    const double rs = ls_extrap_rs(id, ext, j1, j5, j9, dphi, g, iX, iY, iZ);
+    const double rs = ls_extrap_rs(id, ext, geo + 6 * (long long)t, g, iX, iY, iZ);
     const double ev = ext[id];
     if (stage == 0) out[id] = ev - dt * rs;
     else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (ev - dt * rs);
@@ -576,11 +598,8 @@ int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
   const long long N = d.N(), NG = g.N();
   double* P = ctx->ls_buf;
   double* S = P + NG;
-  double* J1 = S + NG;
-  double* J5 = J1 + NG;
-  double* J9 = J5 + NG;
-  double* dphi = J9 + NG;  // 3*NG
-  double* D1 = dphi + 3 * NG;
+  double* geo = S + NG;  // 6 per solid point, at most 6*NG
+  double* D1 = geo + 6 * NG;
   double* D2 = D1 + NG;
   hipStream_t st = ctx->stream;
   const unsigned bg = blocks_for(NG);
@@ -588,18 +607,9 @@ int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
           hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, pres, d.nx, d.ny, d.nz, P, g));
   EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
           hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, sobject, d.nx, d.ny, d.nz, S, g));
-  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
-          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g));
-  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
-          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g));
-  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
-          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g));
   // cuExtrapolation(d_press, d_sobject, d_jbn, deltaX.., NxG.., deltamin*0.5, 1)
   // hands 1.0/deltaX to every kernel (interface.cu:373-421)
   const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz, dt = deltamin * 0.5;
-  EMX_TIMED(ctx, "ls_grad1", 56.0 * NG,
-          hipLaunchKernelGGL(k_ls_grad1, dim3(bg), dim3(256), 0, st, S, J1, J5, J9, g,
-                     iX, iY, iZ, dphi));
   // the solid points, in order; D1 = D2 = P at every point (the fluid
   // points are never written again)
   const int nb = (int)((NG + LS_CHUNK - 1) / LS_CHUNK);
@@ -612,30 +622,35 @@ int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
             hipLaunchKernelGGL(k_ls_scan, dim3(1), dim3(1024), 0, st, bcount, nb, count));
   EMX_TIMED(ctx, "ls_solid_list", 8.0 * NG,
             hipLaunchKernelGGL(k_ls_solid_write, dim3(nb), dim3(256), 0, st, S, NG, bcount, list));
-  EMX_TIMED(ctx, "ls_copy", 24.0 * NG,
-            hipLaunchKernelGGL(k_copy2, dim3(bg), dim3(256), 0, st, P, D1, D2, NG));
-  EMX_TRY_HIP(hipGetLastError());
-  // algorithmic bytes of a stage per solid point (only known on the device;
-  // read back while the context records timings): list entry, ext, three
-  // metrics, three gradient components, P (stages 1, 2), out
+  // algorithmic bytes per solid point (the count is only known on the
+  // device; read back while the context records timings): geometry = list
+  // entry, S, three metrics in, six values out; a stage = list entry, ext,
+  // the six values, P (stages 1, 2), out
   double nsolid = 0.0;
-  if (ctx->timing && (ctx->timing_label.empty() || ctx->timing_label == "ls_extrap_stage")) {
+  if (ctx->timing && (ctx->timing_label.empty() || ctx->timing_label == "ls_extrap_stage" ||
+                      ctx->timing_label == "ls_solid_geom")) {
     int h = 0;
     EMX_TRY_HIP(hipMemcpyAsync(&h, count, sizeof(int), hipMemcpyDeviceToHost, st));
     EMX_TRY_HIP(hipStreamSynchronize(st));
     nsolid = h;
   }
   const unsigned bl = bg < 4096u ? bg : 4096u;
+  EMX_TIMED(ctx, "ls_solid_geom", (4.0 + 8.0 * 10) * nsolid,
+            hipLaunchKernelGGL(k_ls_solid_geom, dim3(bl), dim3(256), 0, st, list, count, S, jbn,
+                               d.nx, d.ny, d.nz, g, iX, iY, iZ, geo));
+  EMX_TIMED(ctx, "ls_copy", 24.0 * NG,
+            hipLaunchKernelGGL(k_copy2, dim3(bg), dim3(256), 0, st, P, D1, D2, NG));
+  EMX_TRY_HIP(hipGetLastError());
   for (int it = 1; it <= 10; it++) {
     EMX_TIMED(ctx, "ls_extrap_stage", (4.0 + 8.0 * 8) * nsolid,
               hipLaunchKernelGGL(k_ls_extrap_list, dim3(bl), dim3(256), 0, st, 0, list, count, P,
-                                 P, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D1));
+                                 P, geo, g, iX, iY, iZ, dt, D1));
     EMX_TIMED(ctx, "ls_extrap_stage", (4.0 + 8.0 * 9) * nsolid,
               hipLaunchKernelGGL(k_ls_extrap_list, dim3(bl), dim3(256), 0, st, 1, list, count, D1,
-                                 P, J1, J5, J9, dphi, g, iX, iY, iZ, dt, D2));
+                                 P, geo, g, iX, iY, iZ, dt, D2));
     EMX_TIMED(ctx, "ls_extrap_stage", (4.0 + 8.0 * 9) * nsolid,
               hipLaunchKernelGGL(k_ls_extrap_list, dim3(bl), dim3(256), 0, st, 2, list, count, D2,
-                                 P, J1, J5, J9, dphi, g, iX, iY, iZ, dt, P));
+                                 P, geo, g, iX, iY, iZ, dt, P));
   }
   EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
           hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g,
