@@ -100,7 +100,7 @@ struct emx_batch {
   std::vector<Box> box;
   // scratch
   double* acc = nullptr;        // sum of N
-  double* ls = nullptr;         // sum of 6 * NG
+  double* ls = nullptr;         // sum of 3 * NG
   std::vector<long long> acc_off, ls_off;
   // line passes, per direction
   emx::BatchShape shape[3];
@@ -116,7 +116,6 @@ struct emx_batch {
   std::vector<emx::LsBatchGrid> ls_h;
   emx::LsBatchGrid* ls_d = nullptr;
   int* blk_pad_d = nullptr;
-  const double** jbn_d = nullptr;
   int nblk_pad = 0;
   // state
   bool dirty = true;
@@ -146,7 +145,6 @@ void free_batch(emx_batch* b) {
   (void)hipFree(b->waves_d);
   (void)hipFree(b->ls_d);
   (void)hipFree(b->blk_pad_d);
-  (void)hipFree(b->jbn_d);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
   if (b->ev_out) (void)hipEventDestroy(b->ev_out);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -266,7 +264,7 @@ int launch_stage(emx_batch* b) {
   return op_batch_ls_step(b->stream, b->ls_d, b->blk_pad_d, b->nblk_pad, b->nbox);
 }
 
-// Descriptors of every phase from the bound boxes; pads the metrics.
+// Descriptors of every phase from the bound boxes.
 int commit(emx_batch* b) {
   for (int i = 0; i < b->nbox; i++)
     if (!b->box[i].bound) {
@@ -275,7 +273,6 @@ int commit(emx_batch* b) {
     }
   for (int d = 0; d < 3; d++) EMX_TRY(plan_lines(b, d, b->shape[d]));
   std::vector<int> bp(b->nbox + 1, 0);
-  std::vector<const double*> jb(b->nbox);
   b->ls_h.resize(b->nbox);
   for (int i = 0; i < b->nbox; i++) {
     const Dims& d = b->dims[i];
@@ -288,24 +285,20 @@ int commit(emx_batch* b) {
     G.nz = d.nz;
     G.phi = B.phi;
     G.vel = B.u;
+    G.jbn = B.jbn;
     G.P = base;
-    G.J = base + NG;
-    G.T1 = base + 4 * NG;
-    G.T2 = base + 5 * NG;
+    G.T1 = base + NG;
+    G.T2 = base + 2 * NG;
     G.iX = 1.0 / B.dxyz[0];
     G.iY = 1.0 / B.dxyz[1];
     G.iZ = 1.0 / B.dxyz[2];
     G.dt = B.dt;
     bp[i + 1] = bp[i] + (int)((NG + 255) / 256);
-    jb[i] = B.jbn;
   }
   b->nblk_pad = bp[b->nbox];
   EMX_TRY_HIP(hipMemcpy(b->ls_d, b->ls_h.data(), b->nbox * sizeof(LsBatchGrid),
                         hipMemcpyHostToDevice));
   EMX_TRY_HIP(hipMemcpy(b->blk_pad_d, bp.data(), bp.size() * sizeof(int), hipMemcpyHostToDevice));
-  EMX_TRY_HIP(hipMemcpy(b->jbn_d, jb.data(), jb.size() * sizeof(double*), hipMemcpyHostToDevice));
-  EMX_TRY(op_batch_ls_pad_metrics(b->stream, b->ls_d, b->blk_pad_d, b->nblk_pad, b->nbox,
-                                  b->jbn_d));
   EMX_TRY_HIP(hipStreamSynchronize(b->stream));
   b->dirty = false;
   return EMX_OK;
@@ -346,7 +339,7 @@ int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme, voi
     b->acc_off.push_back(accN);
     b->ls_off.push_back(lsN);
     accN += d.N();
-    lsN += 6 * emx::ls_padded_points(d.nx, d.ny, d.nz);
+    lsN += 3 * emx::ls_padded_points(d.nx, d.ny, d.nz);
   }
   b->box.resize(nbox);
   if (hipGetDevice(&b->device) != hipSuccess ||
@@ -361,7 +354,6 @@ int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme, voi
             hipMalloc(&b->ls, lsN * sizeof(double)) == hipSuccess &&
             hipMalloc(&b->ls_d, nbox * sizeof(emx::LsBatchGrid)) == hipSuccess &&
             hipMalloc(&b->blk_pad_d, (nbox + 1) * sizeof(int)) == hipSuccess &&
-            hipMalloc(&b->jbn_d, nbox * sizeof(double*)) == hipSuccess &&
             hipMalloc(&b->waves_d, nbox * sizeof(emx::BatchWaveLines<emx::ProbGradDiv>)) ==
                 hipSuccess;
   for (int d = 0; d < 3 && ok; d++)

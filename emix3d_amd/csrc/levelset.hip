@@ -349,11 +349,11 @@ __device__ __forceinline__ void stencil_right(int q, int n, const double bd[6], 
 // back).  vel is the interior velocity (3 components of
 // (nx, ny, nz)): the padded velocity the reference builds is its mirror
 // image, so a padded point reads the interior point at its mirrored
-// coordinates instead of a padded copy.
+// coordinates instead of a padded copy.  The metrics (jbn, 11 components
+// of the interior grid; 1, 5, 9 used) are read the same way.
 __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                                                  const double* P, const double* vel,
-                                                 const double* j1, const double* j5,
-                                                 const double* j9, PDims g, double iX,
+                                                 const double* jbn, PDims g, double iX,
                                                  double iY, double iZ, double dt,
                                                  double* out, double* phi_int, long long id) {
   const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
@@ -363,12 +363,12 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                   z >= g.z - GC))
     return;  // a ghost point of the last stage: not part of the result
   const double ih[3] = {iX, iY, iZ};
-  const double met[3] = {j1[id], j5[id], j9[id]};
   const unsigned nx = (unsigned)(g.x - 2 * GC), ny = (unsigned)(g.y - 2 * GC),
                  nz = (unsigned)(g.z - 2 * GC);
   const unsigned vid = (unsigned)mirror(x, (int)nx) +
                        nx * ((unsigned)mirror(y, (int)ny) + ny * (unsigned)mirror(z, (int)nz));
   const unsigned vN = nx * ny * nz;
+  const double met[3] = {jbn[vid], jbn[(size_t)4 * vN + vid], jbn[(size_t)8 * vN + vid]};
   double r[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -402,14 +402,13 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
 }
 
 __global__ void k_weno_stage(int stage, const double* src, const double* P,
-                             const double* vel, const double* j1,
-                             const double* j5, const double* j9, PDims g,
+                             const double* vel, const double* jbn, PDims g,
                              double iX, double iY, double iZ, double dt,
                              double* out, double* phi_int) {
   const long long NG = g.N();
   for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; id < NG;
        id += (long long)gridDim.x * blockDim.x)
-    weno_stage_point(stage, src, P, vel, j1, j5, j9, g, iX, iY, iZ, dt, out, phi_int, id);
+    weno_stage_point(stage, src, P, vel, jbn, g, iX, iY, iZ, dt, out, phi_int, id);
 }
 
 // ---------------------------------------------------------------------------
@@ -526,28 +525,8 @@ k_batch_weno_stage(const LsBatchGrid* grids, const int* blk_start, int ngrid, in
   if (id >= NG) return;
   const double* src = stage == 0 ? G.P : (stage == 1 ? G.T1 : G.T2);
   double* out = stage == 0 ? G.T1 : G.T2;
-  weno_stage_point(stage, src, G.P, G.vel, G.J, G.J + NG, G.J + 2 * NG, g, G.iX, G.iY, G.iZ,
+  weno_stage_point(stage, src, G.P, G.vel, G.jbn, g, G.iX, G.iY, G.iZ,
                    G.dt, out, stage == 2 ? G.phi : nullptr, id);
-}
-
-__global__ void __launch_bounds__(256)
-k_batch_pad_metrics(const LsBatchGrid* grids, const int* blk_start, int ngrid,
-                    const double* const* jbn) {
-  const int gi = batch_grid(blk_start, ngrid);
-  const LsBatchGrid& G = grids[gi];
-  const PDims g{G.nx + 2 * GC, G.ny + 2 * GC, G.nz + 2 * GC};
-  const long long p = (long long)((int)xcd_block(blockIdx.x, gridDim.x) - blk_start[gi]) * 256 + threadIdx.x;
-  if (p >= g.N()) return;
-  const unsigned up = (unsigned)p, row = up / (unsigned)g.x;
-  const int i = (int)(up - row * (unsigned)g.x), j = (int)(row % (unsigned)g.y),
-            k = (int)(row / (unsigned)g.y);
-  const long long N = (long long)G.nx * G.ny * G.nz, NG = g.N();
-  const long long src = mirror(i, G.nx) + (long long)G.nx * (mirror(j, G.ny) +
-                                                             (long long)G.ny * mirror(k, G.nz));
-  const double* J = jbn[gi];
-  G.J[p] = J[src];
-  G.J[NG + p] = J[4 * N + src];
-  G.J[2 * NG + p] = J[8 * N + src];
 }
 
 unsigned blocks_for(long long n) {
@@ -570,14 +549,6 @@ size_t ls_scratch_doubles(const Dims& d) {
 
 long long ls_padded_points(int nx, int ny, int nz) {
   return (long long)(nx + 2 * GC) * (ny + 2 * GC) * (nz + 2 * GC);
-}
-
-int op_batch_ls_pad_metrics(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
-                            int nblk_pad, int ngrid, const double* const* jbn) {
-  hipLaunchKernelGGL(k_batch_pad_metrics, dim3(nblk_pad), dim3(256), 0, st, grids, blk_pad,
-                     ngrid, jbn);
-  EMX_TRY_HIP(hipGetLastError());
-  return EMX_OK;
 }
 
 int op_batch_ls_step(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
@@ -717,33 +688,24 @@ int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
   const PDims g = padded(d);
   const long long N = d.N(), NG = g.N();
   double* P = ctx->ls_buf;
-  double* J1 = P + NG;
-  double* J5 = J1 + NG;
-  double* J9 = J5 + NG;
-  double* T1 = J9 + NG;
+  double* T1 = P + NG;
   double* T2 = T1 + NG;
   hipStream_t st = ctx->stream;
   const unsigned bg = blocks_for(NG);
   EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
           hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, phi, d.nx, d.ny, d.nz, P, g));
-  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
-          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn, d.nx, d.ny, d.nz, J1, g));
-  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
-          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 4 * N, d.nx, d.ny, d.nz, J5, g));
-  EMX_TIMED(ctx, "ls_pad", 16.0 * NG,
-          hipLaunchKernelGGL(k_pad, dim3(bg), dim3(256), 0, st, jbn + 8 * N, d.nx, d.ny, d.nz, J9, g));
   const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz;
-  // per stage: phi in (and the step's phi), the metrics, phi out on the
-  // padded grid; the velocity once on the interior grid
-  EMX_TIMED(ctx, "ls_weno_stage", 40.0 * NG + 24.0 * N,
-          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, vel, J1,
-                     J5, J9, g, iX, iY, iZ, dt, T1, nullptr));
-  EMX_TIMED(ctx, "ls_weno_stage", 48.0 * NG + 24.0 * N,
-          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, vel, J1,
-                     J5, J9, g, iX, iY, iZ, dt, T2, nullptr));
-  EMX_TIMED(ctx, "ls_weno_stage", 40.0 * NG + 32.0 * N,
-          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, vel, J1,
-                     J5, J9, g, iX, iY, iZ, dt, nullptr, phi));
+  // per stage: phi in (and the step's phi) and phi out on the padded grid;
+  // the velocity and the metrics once on the interior grid
+  EMX_TIMED(ctx, "ls_weno_stage", 16.0 * NG + 48.0 * N,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 0, P, P, vel, jbn, g,
+                     iX, iY, iZ, dt, T1, nullptr));
+  EMX_TIMED(ctx, "ls_weno_stage", 24.0 * NG + 48.0 * N,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, vel, jbn, g,
+                     iX, iY, iZ, dt, T2, nullptr));
+  EMX_TIMED(ctx, "ls_weno_stage", 16.0 * NG + 56.0 * N,
+          hipLaunchKernelGGL(k_weno_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, vel, jbn, g,
+                     iX, iY, iZ, dt, nullptr, phi));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -76,8 +76,8 @@ struct LsBatchGrid {
   int nx, ny, nz;
   double* phi;
   const double* vel;
+  const double* jbn;  // the box's metrics (11 components, interior grid)
   double* P;
-  double* J;
   double* T1;
   double* T2;
   double iX, iY, iZ, dt;
@@ -85,8 +85,6 @@ struct LsBatchGrid {
 long long ls_padded_points(int nx, int ny, int nz);
 // blk_pad: per-grid first workgroup (ngrid + 1 entries) over the padded
 // points, 256 points per workgroup.
-int op_batch_ls_pad_metrics(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
-                            int nblk_pad, int ngrid, const double* const* jbn);
 int op_batch_ls_step(hipStream_t st, const LsBatchGrid* grids, const int* blk_pad,
                      int nblk_pad, int ngrid);
 

@@ -197,7 +197,7 @@ int emx_batch_create(emx_batch** out, int nbox, const int* dims, int scheme,
 int emx_batch_destroy(emx_batch* b);
 int emx_batch_set_stream(emx_batch* b, void* stream);
 /* Device arrays of box i (column-major, as the single-grid entries): u (3
- * comps, in), jbn (11, in; 1/5/9 are padded once per bind), phi (in/out),
+ * comps, in), jbn (11, in; read at every stage like u), phi (in/out),
  * dvel (9, out), div (out); inverse spacings of the derivatives; spacings
  * and time step of the level-set step. */
 int emx_batch_bind(emx_batch* b, int i, const double* u, const double* jbn, double* phi,
