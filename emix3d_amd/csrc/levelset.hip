@@ -487,8 +487,8 @@ __global__ void k_ls_reinit_stage(int stage, const double* src, const double* P,
 // ---------------------------------------------------------------------------
 // The same step for many grids in one launch per phase (the batched stage):
 // workgroup b serves grid gi with blk_start[gi] <= b < blk_start[gi+1],
-// points (b - blk_start[gi])*256 + threadIdx.x of that phase's index space
-// (padded points for pad and the stages, interior points for unpad).
+// points (b - blk_start[gi])*256 + threadIdx.x of the padded grid (the pad
+// and the three stages; the last writes the interior of phi).
 __device__ __forceinline__ int batch_grid(const int* blk_start, int ngrid) {
   const int b = (int)xcd_block(blockIdx.x, gridDim.x);
   int lo = 0, hi = ngrid;
