@@ -9,6 +9,7 @@ per phase over all boxes, replayed from a HIP graph.
     the eager result each time;
   * the wavefront kind along x with each box on its own chunk length (1, 2,
     4 and 8 points per lane in one launch), and its shape checks;
+  * metrics changed in place after the bind are used by the next stage;
   * the autotuner keeps a built shape and the results do not change;
   * the single-grid entry emx_grad_div gives the same fields.
 """
@@ -120,6 +121,22 @@ def test_batch_wave_shape_only_along_x():
             b.set_plan(d, 0, 4)
     with pytest.raises(EmxError):
         b.set_plan(0, 0, 3)  # not built
+
+
+def test_batch_reads_metrics_in_place(oracle):
+    # the metrics are read at every stage, like the velocity: changed in
+    # place after the bind, the next stage uses the new values
+    b, boxes = make_batch(DIMS[:3])
+    reset(boxes)
+    b.stage(graph=False)
+    for z in boxes:
+        z["jbn"][[0, 4, 8]] *= 1.5  # metrics 1, 5, 9
+        z["jbn"][9] *= 1.5          # and J
+    refs = [reference(oracle, z) for z in boxes]
+    for graph in (False, True):
+        reset(boxes)
+        b.stage(graph=graph)
+        check(oracle, boxes, refs, f"metrics changed in place graph={graph}")
 
 
 def test_graph_replays_match_eager(oracle):
