@@ -414,42 +414,42 @@ __global__ void k_weno_stage(int stage, const double* src, const double* P,
 // ---------------------------------------------------------------------------
 // Level-set reinitialisation (cuReinitLsJB, lsTransportEc.cu:422-676).
 
-// Ghost layers of the padded phi refreshed from its interior
-// (cuGhostCellsMirror3d{Z,Y,X}, lsTools.cu:482-585, applied in that order
-// over the full padded ranges, which leaves every ghost point equal to the
-// interior point at its mirrored coordinates), and phi0 = phi.
-__global__ void k_ls_ghost_mirror(double* P, double* P0, PDims g, int nx, int ny, int nz) {
-  const long long NG = g.N();
-  for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-       id < NG; id += (long long)gridDim.x * blockDim.x) {
-    const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
-    const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
-              z = (int)(row / (unsigned)g.y);
-    const long long src = (long long)(mirror(x, nx) + GC) +
-                          (long long)g.x * ((long long)(mirror(y, ny) + GC) +
-                                            (long long)g.y * (mirror(z, nz) + GC));
-    const double v = P[src];  // an interior point: never written here
-    if (src != id) P[id] = v;
-    P0[id] = v;
-  }
+// The padded coordinates whose mirror image (cuGhostCellsMirror3d{Z,Y,X},
+// lsTools.cu:482-585, applied over the full padded ranges) is the interior
+// padded coordinate X of a line of n interior points: X itself, its
+// reflection into the low ghost layers and into the high ones.
+__device__ __forceinline__ int mirror_images(int X, int n, int c[3]) {
+  int k = 0;
+  c[k++] = X;
+  if (X < 2 * GC) c[k++] = 2 * GC - 1 - X;
+  if (X >= n) c[k++] = 2 * (n + GC) - 1 - X;
+  return k;
 }
 
 // One SSP-RK3 stage of the reinitialisation at every padded point: the two
 // JB WENO derivatives per axis, the Godunov |grad phi| by the sign of phi0
 // and rs = S(phi0)(|grad phi| - 1), S = phi0/sqrt(phi0^2 + dmin^2)
 // (reini_RS_WENOJB, lsTools.cu:824-900), then RunGK_{First,Second,Third}S
-// (extrapol.cu:155-212).  src is the stage input, P the iteration's phi;
-// out may alias P only in the last stage.
+// (extrapol.cu:155-212).  src is the stage input, P the iteration's phi,
+// which is also its phi0 (the reference copies phi to phi0 at the start of
+// every iteration).  The last stage computes the interior points and writes
+// each into out, a different array, together with its mirror images in the
+// ghost layers: out is then the next iteration's phi with the ghost
+// mirroring that iteration would start with already applied.
 __global__ void k_ls_reinit_stage(int stage, const double* src, const double* P,
-                                  const double* P0, const double* dmin, const double* j1,
-                                  const double* j5, const double* j9, PDims g, double iX,
-                                  double iY, double iZ, double dt, double* out) {
+                                  const double* dmin, const double* j1, const double* j5,
+                                  const double* j9, PDims g, double iX, double iY, double iZ,
+                                  double dt, double* out) {
   const long long NG = g.N();
+  const double* P0 = P;
   for (long long id = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
        id < NG; id += (long long)gridDim.x * blockDim.x) {
     const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
     const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
               z = (int)(row / (unsigned)g.y);
+    if (stage == 2 && (x < GC || x >= g.x - GC || y < GC || y >= g.y - GC || z < GC ||
+                       z >= g.z - GC))
+      continue;  // a ghost point: written as an interior point's image
     const double ih[3] = {iX, iY, iZ};
     const double met[3] = {j1[id], j5[id], j9[id]};
     const double p0 = P0[id];
@@ -478,9 +478,20 @@ __global__ void k_ls_reinit_stage(int stage, const double* src, const double* P,
     const double so = p0 / sqrt(p0 * p0 + d * d);
     const double rs = 1.0 * so * (grad_mod - 1.0);
     const double sv = src[id];
-    if (stage == 0) out[id] = sv - dt * rs;
-    else if (stage == 1) out[id] = 0.75 * P[id] + 0.25 * (sv - dt * rs);
-    else out[id] = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+    if (stage == 0) {
+      out[id] = sv - dt * rs;
+    } else if (stage == 1) {
+      out[id] = 0.75 * P[id] + 0.25 * (sv - dt * rs);
+    } else {
+      const double o = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+      int cx[3], cy[3], cz[3];
+      const int kx = mirror_images(x, g.x - 2 * GC, cx), ky = mirror_images(y, g.y - 2 * GC, cy),
+                kz = mirror_images(z, g.z - 2 * GC, cz);
+      for (int c = 0; c < kz; c++)
+        for (int b = 0; b < ky; b++)
+          for (int a = 0; a < kx; a++)
+            out[(unsigned)cx[a] + (unsigned)g.x * ((unsigned)cy[b] + (unsigned)g.y * (unsigned)cz[c])] = o;
+    }
   }
 }
 
@@ -635,9 +646,9 @@ int op_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmi
   const Dims d = ctx->d;
   const PDims g = padded(d);
   const long long N = d.N(), NG = g.N();
-  double* P = ctx->ls_buf;
-  double* P0 = P + NG;
-  double* J1 = P0 + NG;
+  double* P = ctx->ls_buf;  // the iteration's phi, ping-pong with Q
+  double* Q = P + NG;
+  double* J1 = Q + NG;
   double* J5 = J1 + NG;
   double* J9 = J5 + NG;
   double* Dm = J9 + NG;
@@ -659,20 +670,21 @@ int op_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmi
                                J9, g));
   // Dev1thO_Downwind receives 1.0/deltaX (lsTransportEc.cu:487-495)
   const double iX = 1.0 / dx, iY = 1.0 / dy, iZ = 1.0 / dz, dt = 0.5 * deltamin;
+  // the padding already mirrors the ghost layers of the first iteration
   for (int it = 1; it <= 10; it++) {
-    EMX_TIMED(ctx, "ls_ghost_mirror", 16.0 * NG,
-              hipLaunchKernelGGL(k_ls_ghost_mirror, dim3(bg), dim3(256), 0, st, P, P0, g, d.nx,
-                                 d.ny, d.nz));
-    // per point: src, three metrics, phi0, dmin, out (and P in stages 1, 2)
+    // per point: src, three metrics, phi (= phi0), dmin, out
     EMX_TIMED(ctx, "ls_reinit_stage", 56.0 * NG,
-              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 0, P, P, P0, Dm,
+              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 0, P, P, Dm,
                                  J1, J5, J9, g, iX, iY, iZ, dt, T1));
-    EMX_TIMED(ctx, "ls_reinit_stage", 64.0 * NG,
-              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, P0, Dm,
+    EMX_TIMED(ctx, "ls_reinit_stage", 56.0 * NG,
+              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 1, T1, P, Dm,
                                  J1, J5, J9, g, iX, iY, iZ, dt, T2));
-    EMX_TIMED(ctx, "ls_reinit_stage", 64.0 * NG,
-              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, P0, Dm,
-                                 J1, J5, J9, g, iX, iY, iZ, dt, P));
+    EMX_TIMED(ctx, "ls_reinit_stage", 56.0 * NG,
+              hipLaunchKernelGGL(k_ls_reinit_stage, dim3(bg), dim3(256), 0, st, 2, T2, P, Dm,
+                                 J1, J5, J9, g, iX, iY, iZ, dt, Q));
+    double* t = P;
+    P = Q;
+    Q = t;
   }
   EMX_TIMED(ctx, "ls_unpad", 16.0 * N,
             hipLaunchKernelGGL(k_unpad, dim3(blocks_for(N)), dim3(256), 0, st, P, g, d.nx, d.ny,
