@@ -946,29 +946,32 @@ int orc_ls_reinit(int nx, int ny, int nz, double *phi, const double *jbn, const 
 
 /* Box test filter of ratio 2: weights (1/4,1/2,1/4) per direction with
  * clamped indices at the walls, applied x, then y, then z. */
-static void box3(int nx, int ny, int nz, const double *in, double *out,
-                 double *tmp)
+/* Separable test filter: weights (ws, 1 - 2 ws, ws) along x, then y, then
+ * z, indices clamped at the walls.  ws = 1/4 is the box filter of ratio 2. */
+static void box3w(int nx, int ny, int nz, const double *in, double *out,
+                  double *tmp, double ws)
 {
+    const double wc = 1.0 - 2.0 * ws;
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
                 int im = i > 0 ? i - 1 : 0, ip = i < nx - 1 ? i + 1 : nx - 1;
-                out[IDX(i, j, k)] = 0.25 * in[IDX(im, j, k)] + 0.5 * in[IDX(i, j, k)]
-                                  + 0.25 * in[IDX(ip, j, k)];
+                out[IDX(i, j, k)] = ws * in[IDX(im, j, k)] + wc * in[IDX(i, j, k)]
+                                  + ws * in[IDX(ip, j, k)];
             }
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
                 int jm = j > 0 ? j - 1 : 0, jp = j < ny - 1 ? j + 1 : ny - 1;
-                tmp[IDX(i, j, k)] = 0.25 * out[IDX(i, jm, k)] + 0.5 * out[IDX(i, j, k)]
-                                  + 0.25 * out[IDX(i, jp, k)];
+                tmp[IDX(i, j, k)] = ws * out[IDX(i, jm, k)] + wc * out[IDX(i, j, k)]
+                                  + ws * out[IDX(i, jp, k)];
             }
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
                 int km = k > 0 ? k - 1 : 0, kp = k < nz - 1 ? k + 1 : nz - 1;
-                out[IDX(i, j, k)] = 0.25 * tmp[IDX(i, j, km)] + 0.5 * tmp[IDX(i, j, k)]
-                                  + 0.25 * tmp[IDX(i, j, kp)];
+                out[IDX(i, j, k)] = ws * tmp[IDX(i, j, km)] + wc * tmp[IDX(i, j, k)]
+                                  + ws * tmp[IDX(i, j, kp)];
             }
 }
 
@@ -976,8 +979,9 @@ static void box3(int nx, int ny, int nz, const double *in, double *out,
  * filter, L_ij = hat(u_i u_j) - hat(u_i) hat(u_j),
  * M_ij = 2 delta^2 (hat(|S| S_ij) - 4 |hat S| hat S_ij), hat S = hat(S),
  * |S| = sqrt(2 S_ij S_ij); C = 0 where <M:M> = 0. */
-int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gp,
-                const double *delta, double *nut)
+int orc_dynsmag_cfg(int nx, int ny, int nz, const double *u, const double *gp,
+                    const double *delta, double *nut, double ratio2, double ws,
+                    double offw)
 {
     long N = (long)nx * ny * nz;
     /* fields: S(6) |S| hatS(6) |hatS| SS-hat(6) uu-hat(6) uhat(3) LM MM tmp out */
@@ -993,38 +997,38 @@ int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gp,
         for (int q = 0; q < 6; q++) {
             double s = 0.5 * (gp[(3 * ia[q] + ib[q]) * N + p] + gp[(3 * ib[q] + ia[q]) * N + p]);
             S[q * N + p] = s;
-            s2 += (q < 3 ? 1.0 : 2.0) * s * s;
+            s2 += (q < 3 ? 1.0 : offw) * s * s;
         }
         aS[p] = sqrt(2.0 * s2);
     }
     for (int q = 0; q < 6; q++) {
-        box3(nx, ny, nz, S + q * N, hS + q * N, tmp);
+        box3w(nx, ny, nz, S + q * N, hS + q * N, tmp, ws);
         for (long p = 0; p < N; p++) t1[p] = aS[p] * S[q * N + p];
-        box3(nx, ny, nz, t1, hSS + q * N, tmp);
+        box3w(nx, ny, nz, t1, hSS + q * N, tmp, ws);
         for (long p = 0; p < N; p++) t1[p] = u[ia[q] * N + p] * u[ib[q] * N + p];
-        box3(nx, ny, nz, t1, huu + q * N, tmp);
+        box3w(nx, ny, nz, t1, huu + q * N, tmp, ws);
     }
-    for (int c = 0; c < 3; c++) box3(nx, ny, nz, u + c * N, hu + c * N, tmp);
+    for (int c = 0; c < 3; c++) box3w(nx, ny, nz, u + c * N, hu + c * N, tmp, ws);
     for (long p = 0; p < N; p++) {
         double s2 = 0.0;
         for (int q = 0; q < 6; q++)
-            s2 += (q < 3 ? 1.0 : 2.0) * hS[q * N + p] * hS[q * N + p];
+            s2 += (q < 3 ? 1.0 : offw) * hS[q * N + p] * hS[q * N + p];
         ahS[p] = sqrt(2.0 * s2);
     }
     for (long p = 0; p < N; p++) {
         double lm = 0.0, mm = 0.0, d2 = delta[p] * delta[p];
         for (int q = 0; q < 6; q++) {
             double L = huu[q * N + p] - hu[ia[q] * N + p] * hu[ib[q] * N + p];
-            double M = 2.0 * d2 * (hSS[q * N + p] - 4.0 * ahS[p] * hS[q * N + p]);
-            double wq = q < 3 ? 1.0 : 2.0;
+            double M = 2.0 * d2 * (hSS[q * N + p] - ratio2 * ahS[p] * hS[q * N + p]);
+            double wq = q < 3 ? 1.0 : offw;
             lm += wq * L * M;
             mm += wq * M * M;
         }
         LM[p] = lm;
         MM[p] = mm;
     }
-    box3(nx, ny, nz, LM, t1, tmp);
-    box3(nx, ny, nz, MM, t2, tmp);
+    box3w(nx, ny, nz, LM, t1, tmp, ws);
+    box3w(nx, ny, nz, MM, t2, tmp, ws);
     for (long p = 0; p < N; p++) {
         double C = t2[p] > 0.0 ? t1[p] / t2[p] : 0.0;
         double v = C * delta[p] * delta[p] * aS[p];
@@ -1032,6 +1036,12 @@ int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gp,
     }
     free(w);
     return 0;
+}
+
+int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gp,
+                const double *delta, double *nut)
+{
+    return orc_dynsmag_cfg(nx, ny, nz, u, gp, delta, nut, 4.0, 0.25, 2.0);
 }
 
 

@@ -27,7 +27,13 @@
  *       untouched fluid points, WENO5 exactness on cubics and 5th-order
  *       convergence, 1-D upwind reduction).
  *   orc_dynsmag: no reference implementation exists (the reference's
- *       model is the structure-function sgdm); pinned only by closed forms.
+ *       model is the structure-function sgdm); pinned by closed forms
+ *       (zero for uniform flow and rigid rotation; on any linear field
+ *       u = A x of a grid with spacings h_a, nu_t = -(A H^2 A^T : S) /
+ *       (12 S:S) away from the walls, H = diag(h_a), independent of the
+ *       filter width) that fail for a wrong filter ratio, filter weight or
+ *       off-diagonal weight, and by the model's definition written with full
+ *       3x3 tensors and a 27-point filter on small random fields.
  *   orc_*_adj: the reference has no adjoint.  Each is the transpose of the
  *       oracle function it names, pinned by dot-product tests against that
  *       function (itself pinned as above) and by the transposed dense
@@ -126,6 +132,14 @@ void orc_pad_mirror(int nx, int ny, int nz, int g, const double *in,
  * grid filter width delta (one value per point).  Clipped at nu_t >= 0. */
 int orc_dynsmag(int nx, int ny, int nz, const double *u, const double *gphys,
                 const double *delta, double *nut);
+/* The same model with its constants as arguments: ratio2 = (test filter /
+ * grid filter width)^2 (4 for ratio 2), ws = side weight of the separable
+ * test filter (ws, 1 - 2 ws, ws) (1/4), offw = weight of an off-diagonal
+ * pair in the tensor contractions (2).  orc_dynsmag is (4, 1/4, 2); tests
+ * pass other values to show that the closed-form checks fail on them. */
+int orc_dynsmag_cfg(int nx, int ny, int nz, const double *u, const double *gphys,
+                    const double *delta, double *nut, double ratio2, double ws,
+                    double offw);
 
 /* Level-set reinitialisation cuReinitLsJB (src/LevelSet/lsTransportEc.cu:
  * 422-676): ten iterations of ghost mirroring (cuGhostCellsMirror3d{Z,Y,X},

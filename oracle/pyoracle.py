@@ -61,6 +61,7 @@ def lib():
             "orc_pad_mirror": [_I, _I, _I, _I, _P, _P],
             "orc_weno5_derivs": [_I, _I, _I, _P, _P, _D, _D, _D, _P, _P],
             "orc_dynsmag": [_I, _I, _I, _P, _P, _P, _P],
+            "orc_dynsmag_cfg": [_I, _I, _I, _P, _P, _P, _P, _D, _D, _D],
             "orc_rk_update": [_I, _L, _P, _P, _P, _P, _P, _D],
             "orc_frontera_val": [_I, _I, _I, _P],
             "orc_ls_ghost_mirror": [_I, _I, _I, _I, _P],
@@ -409,11 +410,19 @@ def pad_mirror(a, g=5):
     return out
 
 
-def dynsmag(u, gphys, delta):
+def dynsmag(u, gphys, delta, ratio2=None, ws=None, offw=None):
+    """Dynamic Smagorinsky nu_t; ratio2, ws, offw override the model's
+    constants (4, 1/4, 2) to plant mistakes in tests."""
     nx, ny, nz = dims(u)
     nut = np.empty((nz, ny, nx))
-    _check(lib().orc_dynsmag(nx, ny, nz, _ptr(u), _ptr(gphys), _ptr(delta), _ptr(nut)),
-           "dynsmag")
+    u, gphys, delta = (np.ascontiguousarray(a, dtype=np.float64) for a in (u, gphys, delta))
+    if ratio2 is None and ws is None and offw is None:
+        rc = lib().orc_dynsmag(nx, ny, nz, _ptr(u), _ptr(gphys), _ptr(delta), _ptr(nut))
+    else:
+        rc = lib().orc_dynsmag_cfg(nx, ny, nz, _ptr(u), _ptr(gphys), _ptr(delta), _ptr(nut),
+                                   4.0 if ratio2 is None else ratio2,
+                                   0.25 if ws is None else ws, 2.0 if offw is None else offw)
+    _check(rc, "dynsmag")
     return nut
 
 
