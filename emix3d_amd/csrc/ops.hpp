@@ -50,6 +50,11 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
 
 int op_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
                const double* delta, double* nut, double c4, double* vis);
+// The dynamic model's two kernels on a stream, with LM, MM (N doubles each)
+// as scratch; ctx (may be NULL) only for the per-launch timing.
+int launch_dynsmag(emx_ctx* ctx, hipStream_t stream, const Dims& d, const double* u,
+                   const double* dvel, const double* jbn, const double* delta, double* nut,
+                   double c4, double* vis, double* LM, double* MM);
 
 // learned closure (closure.hip); packed operands in ctx->mlp_pack
 size_t mlp_pack_elems();

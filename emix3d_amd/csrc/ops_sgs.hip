@@ -213,153 +213,195 @@ int op_sgdm(emx_ctx* ctx, const double* u, const double* dvel,
 // closure.  Physical gradient on the diagonal-metric grid:
 // du_a/dx_b = m_b * dvel(3b+a) with m = jbn 1, 5, 9.  Test filter: the box
 // filter of ratio 2, weights (1/4, 1/2, 1/4) per direction with clamped
-// indices at the walls, summed x innermost, then y, then z (the separable
-// passes x, y, z in that order).
+// indices at the walls, applied along x, then y, then z.
+//
+// Two kernels, both on the plane march of k_sgs_fused: a workgroup owns a
+// 32 x 8 (i, j) tile and a slab of k planes.  For each plane (and one
+// clamped plane either side of the slab) it forms the filtered quantities
+// at the tile and its one-point halo into LDS, filters them along x and y
+// from LDS, and filters along z in registers as the planes pass:
+//   Z(k) = (1/4 Y(k-1) + 1/2 Y(k)) + 1/4 Y(k+1),
+// the term of plane k+1 added when that plane arrives, so every quantity
+// of a point is formed once per plane and read once from memory.
+//   k_dynsmag_lm:  the 21 quantities S (6), |S| S (6), u_a u_b (6), u (3)
+//                  -> L:M and M:M (two fields of scratch);
+//   k_dynsmag_nut: the filtered L:M and M:M -> C, nu_t, vis.
+constexpr int DS_TX = 32, DS_TY = 8, DS_HX = DS_TX + 2, DS_HY = DS_TY + 2,
+              DS_HP = DS_HX * DS_HY, DS_KS = 16;
 
-__device__ __forceinline__ void sym_strain(const double* dvel, const double* jbn,
-                                           long long N, long long p, double S[6],
-                                           double* aS) {
-  const double m[3] = {jbn[p], jbn[4 * N + p], jbn[8 * N + p]};
+// Strain-rate pairs (0,0) (1,1) (2,2) (0,1) (0,2) (1,2); off-diagonal pairs
+// count twice in contractions.
+__device__ __forceinline__ void strain(const double* dvel, const double* jbn, long long N,
+                                       long long q, double S[6], double* aS) {
+  const double m[3] = {jbn[q], jbn[4 * N + q], jbn[8 * N + q]};
   double g[9];
 #pragma unroll
   for (int a = 0; a < 3; a++)
 #pragma unroll
-    for (int b = 0; b < 3; b++) g[3 * a + b] = m[b] * dvel[(3 * b + a) * N + p];
-  // pairs (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
+    for (int b = 0; b < 3; b++) g[3 * a + b] = m[b] * dvel[(3 * b + a) * N + q];
   const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
   double s2 = 0.0;
 #pragma unroll
-  for (int q = 0; q < 6; q++) {
-    S[q] = 0.5 * (g[3 * ia[q] + ib[q]] + g[3 * ib[q] + ia[q]]);
-    s2 += (q < 3 ? 1.0 : 2.0) * S[q] * S[q];
+  for (int t = 0; t < 6; t++) {
+    S[t] = 0.5 * (g[3 * ia[t] + ib[t]] + g[3 * ib[t] + ia[t]]);
+    s2 += (t < 3 ? 1.0 : 2.0) * S[t] * S[t];
   }
   *aS = sqrt(2.0 * s2);
 }
 
-// Box filter of NQ quantities formed per point by f(p, out[NQ]).
-template <int NQ, class F>
-__device__ __forceinline__ void box27(const Dims& d, int i, int j, int k, F f, double out[NQ]) {
+// The plane march shared by both kernels.  form(q, v) writes the NQ
+// quantities of grid point q; done(p, Z) receives the filtered quantities
+// of an owned point p of the slab.  Planes k0-1 .. k1 are formed (clamped
+// into the grid); when plane kk arrives, Z(kk-1) is completed and handed
+// to done(), Z(kk) gets its middle term and Z(kk+1) its first.
+template <int NQ, class Form, class Done>
+__device__ __forceinline__ void dyn_march(const Dims& d, double (*Q)[DS_HP], Form form,
+                                          Done done) {
   const long long sy = d.nx, sz = (long long)d.nx * d.ny;
-  const int xs[3] = {i > 0 ? i - 1 : 0, i, i < d.nx - 1 ? i + 1 : d.nx - 1};
-  const int ys[3] = {j > 0 ? j - 1 : 0, j, j < d.ny - 1 ? j + 1 : d.ny - 1};
-  const int zs[3] = {k > 0 ? k - 1 : 0, k, k < d.nz - 1 ? k + 1 : d.nz - 1};
-  const double w[3] = {0.25, 0.5, 0.25};
-  double Z[NQ];
+  const int tx = threadIdx.x % DS_TX, ty = threadIdx.x / DS_TX;
+  const unsigned nbx = (d.nx + DS_TX - 1) / DS_TX, nby = (d.ny + DS_TY - 1) / DS_TY;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int i0 = (int)(b % nbx) * DS_TX, j0 = (int)((b / nbx) % nby) * DS_TY;
+  const int k0 = (int)(b / (nbx * nby)) * DS_KS, k1 = min(k0 + DS_KS, d.nz);
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = i < d.nx && j < d.ny;
+  const int hc = (ty + 1) * DS_HX + (tx + 1);
+  double Zn[NQ] = {}, Zc[NQ] = {};  // Z(kk+1) after its first term, Z(kk) after two
+  for (int kk = k0 - 1; kk <= k1; kk++) {
+    const int gk = kk < 0 ? 0 : (kk >= d.nz ? d.nz - 1 : kk);
+    for (int h = threadIdx.x; h < DS_HP; h += blockDim.x) {
+      int gi = i0 - 1 + h % DS_HX, gj = j0 - 1 + h / DS_HX;
+      gi = gi < 0 ? 0 : (gi >= d.nx ? d.nx - 1 : gi);
+      gj = gj < 0 ? 0 : (gj >= d.ny ? d.ny - 1 : gj);
+      double v[NQ];
+      form(gi + gj * sy + gk * sz, v);
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    double Y[NQ];
-#pragma unroll
-    for (int b = 0; b < 3; b++) {
-      double v0[NQ], v1[NQ], v2[NQ];
-      const long long row = ys[b] * sy + zs[c] * sz;
-      f(row + xs[0], v0);
-      f(row + xs[1], v1);
-      f(row + xs[2], v2);
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        const double X = 0.25 * v0[q] + 0.5 * v1[q] + 0.25 * v2[q];
-        Y[q] = b == 0 ? w[0] * X : Y[q] + w[b] * X;
-      }
+      for (int t = 0; t < NQ; t++) Q[t][h] = v[t];
     }
+    __syncthreads();
+    if (own) {
+      double Zd[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; q++) Z[q] = c == 0 ? w[0] * Y[q] : Z[q] + w[c] * Y[q];
+      for (int t = 0; t < NQ; t++) {
+        double X[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          const int c = hc + (r - 1) * DS_HX;
+          X[r] = 0.25 * Q[t][c - 1] + 0.5 * Q[t][c] + 0.25 * Q[t][c + 1];
+        }
+        const double Y = 0.25 * X[0] + 0.5 * X[1] + 0.25 * X[2];
+        Zd[t] = Zc[t] + 0.25 * Y;     // Z(kk-1), valid for kk > k0
+        Zc[t] = Zn[t] + 0.5 * Y;      // Z(kk), valid for kk >= k0
+        Zn[t] = 0.25 * Y;             // Z(kk+1)
+      }
+      if (kk > k0) done(i + j * sy + (long long)(kk - 1) * sz, Zd);
+    }
+    __syncthreads();  // Q is rewritten for the next plane
   }
-#pragma unroll
-  for (int q = 0; q < NQ; q++) out[q] = Z[q];
 }
 
-// Germano-Lilly contractions at each point: with hat() the box filter,
+// L:M and M:M at each point: with hat() the test filter,
 //   L_ij = hat(u_i u_j) - hat(u_i) hat(u_j),
-//   M_ij = 2 delta^2 (hat(|S| S_ij) - 4 |hat S| hat S_ij),
-//   LM = L:M, MM = M:M (off-diagonal pairs counted twice).
+//   M_ij = 2 delta^2 (hat(|S| S_ij) - 4 |hat S| hat S_ij).
 __global__ void __launch_bounds__(256)
 k_dynsmag_lm(const double* u, const double* dvel, const double* jbn, const double* delta,
              Dims d, double* LM, double* MM) {
+  __shared__ double Q[21][DS_HP];
   const long long N = d.N();
   const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
-       p += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(p % d.nx), j = (int)((p / d.nx) % d.ny),
-              k = (int)(p / ((long long)d.nx * d.ny));
-    double hS[6], hSS[6], huu[6], hu[3];
-    box27<6>(d, i, j, k, [&](long long q, double* v) {
-      double a;
-      sym_strain(dvel, jbn, N, q, v, &a);
-    }, hS);
-    box27<6>(d, i, j, k, [&](long long q, double* v) {
-      double S[6], a;
-      sym_strain(dvel, jbn, N, q, S, &a);
+  dyn_march<21>(
+      d, Q,
+      [&](long long q, double* v) {
+        double S[6], aS;
+        strain(dvel, jbn, N, q, S, &aS);
+        const double uq[3] = {u[q], u[N + q], u[2 * N + q]};
 #pragma unroll
-      for (int t = 0; t < 6; t++) v[t] = a * S[t];
-    }, hSS);
-    box27<6>(d, i, j, k, [&](long long q, double* v) {
+        for (int t = 0; t < 6; t++) {
+          v[t] = S[t];
+          v[6 + t] = aS * S[t];
+          v[12 + t] = uq[ia[t]] * uq[ib[t]];
+        }
 #pragma unroll
-      for (int t = 0; t < 6; t++) v[t] = u[ia[t] * N + q] * u[ib[t] * N + q];
-    }, huu);
-    box27<3>(d, i, j, k, [&](long long q, double* v) {
+        for (int c = 0; c < 3; c++) v[18 + c] = uq[c];
+      },
+      [&](long long p, const double* Z) {
+        double s2 = 0.0;
 #pragma unroll
-      for (int t = 0; t < 3; t++) v[t] = u[t * N + q];
-    }, hu);
-    double s2 = 0.0;
+        for (int t = 0; t < 6; t++) s2 += (t < 3 ? 1.0 : 2.0) * Z[t] * Z[t];
+        const double ahS = sqrt(2.0 * s2);
+        const double d2 = delta[p] * delta[p];
+        double lm = 0.0, mm = 0.0;
 #pragma unroll
-    for (int q = 0; q < 6; q++) s2 += (q < 3 ? 1.0 : 2.0) * hS[q] * hS[q];
-    const double ahS = sqrt(2.0 * s2);
-    const double d2 = delta[p] * delta[p];
-    double lm = 0.0, mm = 0.0;
-#pragma unroll
-    for (int q = 0; q < 6; q++) {
-      const double L = huu[q] - hu[ia[q]] * hu[ib[q]];
-      const double M = 2.0 * d2 * (hSS[q] - 4.0 * ahS * hS[q]);
-      const double wq = q < 3 ? 1.0 : 2.0;
-      lm += wq * L * M;
-      mm += wq * M * M;
-    }
-    LM[p] = lm;
-    MM[p] = mm;
-  }
+        for (int t = 0; t < 6; t++) {
+          const double L = Z[12 + t] - Z[18 + ia[t]] * Z[18 + ib[t]];
+          const double M = 2.0 * d2 * (Z[6 + t] - 4.0 * ahS * Z[t]);
+          const double wq = t < 3 ? 1.0 : 2.0;
+          lm += wq * L * M;
+          mm += wq * M * M;
+        }
+        LM[p] = lm;
+        MM[p] = mm;
+      });
 }
 
-// nu_t = max(0, C delta^2 |S|) with C = hat(LM) / hat(MM) (0 where
-// hat(MM) <= 0).
+// nu_t = max(0, C delta^2 |S|) with C = hat(L:M) / hat(M:M) (0 where
+// hat(M:M) <= 0); vis = c4 + nu_t when vis is not NULL.
 __global__ void __launch_bounds__(256)
 k_dynsmag_nut(const double* LM, const double* MM, const double* dvel, const double* jbn,
               const double* delta, Dims d, double* nut, double c4, double* vis) {
+  __shared__ double Q[2][DS_HP];
   const long long N = d.N();
-  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
-       p += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(p % d.nx), j = (int)((p / d.nx) % d.ny),
-              k = (int)(p / ((long long)d.nx * d.ny));
-    double h[2];
-    box27<2>(d, i, j, k, [&](long long q, double* v) {
-      v[0] = LM[q];
-      v[1] = MM[q];
-    }, h);
-    double S[6], aS;
-    sym_strain(dvel, jbn, N, p, S, &aS);
-    const double C = h[1] > 0.0 ? h[0] / h[1] : 0.0;
-    const double v = C * delta[p] * delta[p] * aS;
-    const double nu = v > 0.0 ? v : 0.0;
-    nut[p] = nu;
-    if (vis) vis[p] = c4 * 1.0 + nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
+  dyn_march<2>(
+      d, Q,
+      [&](long long q, double* v) {
+        v[0] = LM[q];
+        v[1] = MM[q];
+      },
+      [&](long long p, const double* Z) {
+        double S[6], aS;
+        strain(dvel, jbn, N, p, S, &aS);
+        const double C = Z[1] > 0.0 ? Z[0] / Z[1] : 0.0;
+        const double v = C * delta[p] * delta[p] * aS;
+        const double nu = v > 0.0 ? v : 0.0;
+        nut[p] = nu;
+        if (vis) vis[p] = c4 * 1.0 + nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
+      });
+}
+
+// Launches the two kernels; LM and MM are 2*N doubles of scratch.
+int launch_dynsmag(emx_ctx* ctx, hipStream_t stream, const Dims& d, const double* u,
+                   const double* dvel, const double* jbn, const double* delta, double* nut,
+                   double c4, double* vis, double* LM, double* MM) {
+  const long long N = d.N();
+  const unsigned tiles = (unsigned)((d.nx + DS_TX - 1) / DS_TX) *
+                         (unsigned)((d.ny + DS_TY - 1) / DS_TY) *
+                         (unsigned)((d.nz + DS_KS - 1) / DS_KS);
+  // reads u(3), dvel(9), jbn(3), delta; writes LM, MM / reads LM, MM,
+  // dvel(9), jbn(3), delta; writes nut (and vis)
+  auto lm = [&] {
+    hipLaunchKernelGGL(k_dynsmag_lm, dim3(tiles), dim3(256), 0, stream, u, dvel, jbn, delta, d,
+                       LM, MM);
+  };
+  auto nu = [&] {
+    hipLaunchKernelGGL(k_dynsmag_nut, dim3(tiles), dim3(256), 0, stream, LM, MM, dvel, jbn,
+                       delta, d, nut, c4, vis);
+  };
+  if (ctx) {
+    EMX_TIMED(ctx, "dynsmag_lm", 18.0 * 8.0 * N, lm());
+    EMX_TIMED(ctx, "dynsmag_nut", (vis ? 17.0 : 16.0) * 8.0 * N, nu());
+  } else {
+    lm();
+    nu();
   }
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
 }
 
 int op_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
                const double* delta, double* nut, double c4, double* vis) {
-  const Dims d = ctx->d;
-  const long long N = d.N();
-  double* LM = ctx->sgs_tmp;
-  double* MM = ctx->sgs_tmp + N;
-  // reads u(3), dvel(9), jbn(3), delta; writes LM, MM / reads LM, MM,
-  // dvel(9), jbn(3), delta; writes nut
-  EMX_TIMED(ctx, "dynsmag_lm", 18.0 * 8.0 * N,
-            hipLaunchKernelGGL(k_dynsmag_lm, dim3(pointwise_blocks(N)), dim3(256), 0,
-                               ctx->stream, u, dvel, jbn, delta, d, LM, MM));
-  EMX_TIMED(ctx, "dynsmag_nut", 16.0 * 8.0 * N,
-            hipLaunchKernelGGL(k_dynsmag_nut, dim3(pointwise_blocks(N)), dim3(256), 0,
-                               ctx->stream, LM, MM, dvel, jbn, delta, d, nut, c4, vis));
-  EMX_TRY_HIP(hipGetLastError());
-  return EMX_OK;
+  const long long N = ctx->d.N();
+  return launch_dynsmag(ctx, ctx->stream, ctx->d, u, dvel, jbn, delta, nut, c4, vis,
+                        ctx->sgs_tmp, ctx->sgs_tmp + N);
 }
 
 }  // namespace emx

@@ -112,7 +112,10 @@ def gdot(a, b):
 
 
 def test_dot_product_128_stretched():
-    """<w, D v> = <D^T w, v> at 128^3 (tanh stretching), every direction."""
+    """<w, D v> = <D^T w, v> at 128^3 for the derivative along every
+    direction (metric-free), and for the divergence with the metrics of the
+    tanh-stretched grid (beta = 2): <(rs_bar, rsv_bar), div(flux)> =
+    <div^T(rs_bar, rsv_bar), flux>."""
     n = 128
     op = Operator(n, n, n, "c6c")
     gen = torch.Generator(device="cuda").manual_seed(20261206)
@@ -122,6 +125,15 @@ def test_dot_product_128_stretched():
         lhs = gdot(w, op.derivative(v, d))
         rhs = gdot(op.derivative_adj(w, d), v)
         report(f"dot dir{d}", abs(lhs - rhs) / max(abs(lhs), abs(rhs)), 1e-13)
+    jbn = op.jacobeano(fields.grid_tanh(n, n, n, device="cuda"))
+    assert float(jbn[0].max() / jbn[0].min()) > 2.0  # stretched
+    flux = torch.randn((6, n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    rb = torch.randn((n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    rvb = torch.randn((n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    rs, rsv = op.divergencia(flux, jbn)
+    lhs = gdot(rb, rs) + gdot(rvb, rsv)
+    rhs = gdot(op.divergencia_adj(rb, rvb, jbn), flux)
+    report("dot divergence (stretched metrics)", abs(lhs - rhs) / max(abs(lhs), abs(rhs)), 1e-13)
 
 
 def composed(op, jbn, x):
