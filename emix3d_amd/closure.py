@@ -100,11 +100,16 @@ class LearnedClosure:
             np.asarray(params, np.float32), device=self.device).contiguous()
         self.norm = None if norm is None else torch.as_tensor(
             np.asarray(norm, np.float32), device=self.device).contiguous()
+        self.static_c = None
 
     @classmethod
     def load(cls, path=WEIGHTS, device="cuda"):
+        """The network, and the static Smagorinsky constant fitted to the
+        same training targets (static_c, None in older files)."""
         z = np.load(path, allow_pickle=False)
-        return cls(z["params"], z["norm"], device)
+        cl = cls(z["params"], z["norm"], device)
+        cl.static_c = float(z["static_c"]) if "static_c" in z.files else None
+        return cl
 
     def save(self, path=WEIGHTS, **meta):
         np.savez(path, params=self.params.cpu().numpy(), norm=self.norm.cpu().numpy(),
@@ -189,17 +194,57 @@ def rel_l2(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
-def evaluate(fam, closure, seeds):
+def strain_scale(dvel, jbn, delta):
+    """delta^2 |S| with |S| = sqrt(2 S_ij S_ij) of the physical gradient
+    du_a/dx_b = jbn(1+4b) dvel(3b+a): the static Smagorinsky model is
+    nu_t = C * delta^2 |S|."""
+    m = (jbn[0], jbn[4], jbn[8])
+    s2 = torch.zeros_like(delta)
+    for a in range(3):
+        for b in range(3):
+            sab = 0.5 * (m[b] * dvel[3 * b + a] + m[a] * dvel[3 * a + b])
+            s2 += sab * sab
+    return delta * delta * torch.sqrt(2.0 * s2)
+
+
+class StaticSmagorinsky:
+    """nu_t = C delta^2 |S| with one constant C, fitted by least squares to
+    the dynamic-Smagorinsky targets of the training snapshots: the baseline
+    a network fed the whole velocity gradient has to beat."""
+
+    def __init__(self, c=None):
+        self.c = c
+
+    def fit(self, fams, per_family=8):
+        num = den = 0.0
+        for fam in fams:
+            for s in fam.seeds[:per_family]:
+                _, dvel, nut = fam.snapshot(s)
+                q = strain_scale(dvel, fam.jbn, fam.delta)
+                num += float((q * nut).sum())
+                den += float((q * q).sum())
+        self.c = num / den
+        return self
+
+    def nut(self, dvel, jbn, delta):
+        return self.c * strain_scale(dvel, jbn, delta)
+
+
+def evaluate(fam, closure, seeds, static=None):
     """Relative L2 errors against dynamic Smagorinsky on the snapshots
     seeds: of nu_t and of the viscous flux divergence rsv (equations 2..4),
-    for the learned closure and for the structure-function sgdm."""
-    out = {"mlp_nut": [], "mlp_rsv": [], "sgdm_nut": [], "sgdm_rsv": []}
+    for the learned closure, for the static Smagorinsky baseline (when
+    given) and for the structure-function sgdm."""
+    names = ["mlp", "sgdm"] + (["static"] if static is not None else [])
+    out = {f"{n}_{q}": [] for n in names for q in ("nut", "rsv")}
     for s in seeds:
         u, dvel, target = fam.snapshot(s)
         rsv_t = fam.rsv(u, dvel, target)[1:4]
-        nut = closure.nut(fam.op, dvel, fam.jbn, fam.delta)
-        amut, _, _ = fam.op.viscosidad_t(u, dvel, fam.jbn, fam.sgd, C4, want_vis5=False)
-        for name, nu in (("mlp", nut), ("sgdm", amut)):
+        models = {"mlp": closure.nut(fam.op, dvel, fam.jbn, fam.delta)}
+        models["sgdm"], _, _ = fam.op.viscosidad_t(u, dvel, fam.jbn, fam.sgd, C4, want_vis5=False)
+        if static is not None:
+            models["static"] = static.nut(dvel, fam.jbn, fam.delta)
+        for name, nu in models.items():
             out[f"{name}_nut"].append(rel_l2(nu, target))
             out[f"{name}_rsv"].append(rel_l2(fam.rsv(u, dvel, nu)[1:4], rsv_t))
     return {k: float(np.mean(v)) for k, v in out.items()}
@@ -229,16 +274,18 @@ def main():
     fams = training_families()
     cl = LearnedClosure()
     hist = cl.train(fams, steps=args.steps, batch=args.batch, log=log)
+    static = StaticSmagorinsky().fit(fams)
+    log(f"static Smagorinsky constant (least squares on the training targets): {static.c:.5f}")
     evals = {}
     held = {"sines128": (fams[0], range(201, 205)),
             "turbulent128": (SnapshotFamily(128, kind="turbulent"), range(1101, 1103)),
             "turbulent256": (SnapshotFamily(256, kind="turbulent"), range(2101, 2103))}
     for name, (fam, seeds) in held.items():
-        ev = evaluate(fam, cl, seeds)
+        ev = evaluate(fam, cl, seeds, static)
         log(f"held-out {name} {list(seeds)}: {ev}")
         for k, v in ev.items():
             evals[f"{name}_{k}"] = v
-    cl.save(args.out, steps=args.steps, loss=np.asarray(hist),
+    cl.save(args.out, steps=args.steps, loss=np.asarray(hist), static_c=static.c,
             eval_keys=np.asarray(list(evals.keys())), eval_vals=np.asarray(list(evals.values())))
     log(f"saved {args.out}")
 

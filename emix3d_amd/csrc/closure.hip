@@ -19,18 +19,10 @@
 // per point and reduces the weight gradients per block, then over blocks.
 #include <algorithm>
 
+#include "mlp_infer.hpp"
 #include "ops_common.hpp"
 
 namespace emx {
-
-constexpr int MLP_IN = 10, MLP_H = 64;
-constexpr int MW1 = 0, MB1 = MW1 + MLP_H * MLP_IN, MW2 = MB1 + MLP_H,
-              MB2 = MW2 + MLP_H * MLP_H, MW3 = MB2 + MLP_H, MB3 = MW3 + MLP_H * MLP_H,
-              MW4 = MB3 + MLP_H, MB4 = MW4 + MLP_H, MNP = MB4 + 1;
-static_assert(MNP == EMX_MLP_NPARAM, "parameter count");
-
-typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 
 __device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu_f(float z) {
@@ -51,13 +43,6 @@ __device__ __forceinline__ void point_features(const double* dvel, const double*
 #pragma unroll
     for (int b = 0; b < 3; b++) f[3 * a + b] = m[b] * dvel[(3 * b + a) * N + p];
   f[9] = delta[p];
-}
-
-// Input neuron of K index k = 8g + j (lane group g, element j) in K-step s
-// of layers 2 and 3: the lane's own result elements of output tiles 2s and
-// 2s+1 (neurons 16t + 4g + i).
-__host__ __device__ __forceinline__ int perm_in(int s, int g, int j) {
-  return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
 }
 
 // Packs the parameters into per-lane BF16 A operands:
@@ -82,102 +67,54 @@ __global__ void k_mlp_pack(const float* P, const float* norm, __bf16* A1, __bf16
   }
 }
 
-// nu_t of every grid point; one wavefront per tile of 16 points.
+// nu_t of every grid point: persistent wavefronts, each taking groups of
+// 64 consecutive points (one per lane, loads coalesced) and keeping the
+// next group's 13 inputs in flight while the network runs on the current.
 __global__ void __launch_bounds__(256)
 k_mlp_infer(const double* dvel, const double* jbn, const double* delta, long long N,
-            const float* P, const float* norm, const __bf16* A1, const __bf16* A2,
-            const __bf16* A3, double* nut, double c4, double* vis) {
-  const int l = threadIdx.x & 63;
-  const int m = l % 16, g = l / 16;
-  // weights and biases of this lane, kept in registers
-  bf16x8 a1[4], a2[4][2], a3[4][2];
+            const float* P, const float* norm, const __bf16* A, double* nut, double c4,
+            double* vis) {
+  __shared__ __attribute__((aligned(16))) float vec[MLP_VEC + 2];
+  __shared__ float xs[4][MLP_IN * 64];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  MlpLaneWeights W;
+  mlp_load_weights(W, A, l);
+  mlp_stage_vectors(vec, P, norm);
+  __syncthreads();
+  const long long ngroup = (N + 63) / 64;
+  const long long nwave = (long long)gridDim.x * (blockDim.x >> 6);
+  long long grp = (long long)blockIdx.x * (blockDim.x >> 6) + w;
+  // raw inputs of one point: dvel 1..9, jbn 1, 5, 9, delta
+  auto load = [&](long long gr, double r[13]) {
+    long long p = gr * 64 + l;
+    p = p < N ? p : N - 1;
 #pragma unroll
-  for (int mt = 0; mt < 4; mt++) {
+    for (int c = 0; c < 9; c++) r[c] = dvel[c * N + p];
+    r[9] = jbn[p];
+    r[10] = jbn[4 * N + p];
+    r[11] = jbn[8 * N + p];
+    r[12] = delta[p];
+  };
+  double nxt[13];
+  if (grp < ngroup) load(grp, nxt);
+  for (; grp < ngroup; grp += nwave) {
+    double cur[13];
 #pragma unroll
-    for (int j = 0; j < 8; j++) a1[mt][j] = A1[(mt * 64 + l) * 8 + j];
+    for (int c = 0; c < 13; c++) cur[c] = nxt[c];
+    if (grp + nwave < ngroup) load(grp + nwave, nxt);
+    float x[MLP_IN];
 #pragma unroll
-    for (int s = 0; s < 2; s++)
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-      for (int j = 0; j < 8; j++) {
-        a2[mt][s][j] = A2[((mt * 2 + s) * 64 + l) * 8 + j];
-        a3[mt][s][j] = A3[((mt * 2 + s) * 64 + l) * 8 + j];
-      }
-  }
-  float bias1[4][4], bias2[4][4], bias3[4][4], w4[4][4];
-#pragma unroll
-  for (int t = 0; t < 4; t++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int o = 16 * t + 4 * g + i;
-      bias1[t][i] = P[MB1 + o];
-      bias2[t][i] = P[MB2 + o];
-      bias3[t][i] = P[MB3 + o];
-      w4[t][i] = P[MW4 + o];
-    }
-  const float b4 = P[MB4], oscale = norm[2 * MLP_IN];
-  const long long ntile = (N + 15) / 16;
-  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long long nwave = ((long long)gridDim.x * blockDim.x) >> 6;
-  for (long long tile = wave; tile < ntile; tile += nwave) {
-    const long long p = tile * 16 + m;
-    const bool valid = p < N;
-    const long long pc = valid ? p : N - 1;
-    // B operand of layer 1: lane group g supplies inputs 8g..8g+7
-    bf16x8 x;
-    if (g < 2) {
-      double f[MLP_IN];
-      point_features(dvel, jbn, delta, N, pc, f);
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        const int k = 8 * g + j;
-        const float v = k < MLP_IN ? (float)((f[k] - norm[k]) * norm[MLP_IN + k]) : 0.0f;
-        x[j] = (__bf16)v;
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < 8; j++) x[j] = (__bf16)0.0f;
-    }
-    float h[4][4];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[t], x, acc, 0, 0, 0);
-#pragma unroll
-      for (int i = 0; i < 4; i++) h[t][i] = silu_f(acc[i] + bias1[t][i]);
-    }
-    // layers 2 and 3
-#pragma unroll
-    for (int layer = 0; layer < 2; layer++) {
-      bf16x8 b[2];
-#pragma unroll
-      for (int s = 0; s < 2; s++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) b[s][j] = (__bf16)(j < 4 ? h[2 * s][j] : h[2 * s + 1][j - 4]);
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 2; s++)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer == 0 ? a2[t][s] : a3[t][s], b[s],
-                                                          acc, 0, 0, 0);
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-          h[t][i] = silu_f(acc[i] + (layer == 0 ? bias2[t][i] : bias3[t][i]));
-      }
-    }
-    // output neuron: FP32 dot product over the 64 neurons of the point,
-    // 16 per lane, summed over the four lane groups
-    float z = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 4; t++)
-#pragma unroll
-      for (int i = 0; i < 4; i++) z += w4[t][i] * h[t][i];
-    z += __shfl_xor(z, 16);
-    z += __shfl_xor(z, 32);
-    if (g == 0 && valid) {
-      const double nu = (double)(oscale * softplus_f(z + b4));
-      nut[p] = nu;
-      if (vis) vis[p] = c4 * 1.0 + nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
+      for (int b = 0; b < 3; b++)
+        x[3 * a + b] = (float)((cur[9 + b] * cur[3 * b + a] - norm[3 * a + b]) *
+                               norm[MLP_IN + 3 * a + b]);
+    x[9] = (float)((cur[12] - norm[9]) * norm[MLP_IN + 9]);
+    const float nu = mlp_group64(W, x, xs[w], vec, l);
+    const long long p = grp * 64 + l;
+    if (p < N) {
+      nut[p] = (double)nu;
+      if (vis) vis[p] = c4 * 1.0 + (double)nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
     }
   }
 }
@@ -368,19 +305,18 @@ size_t mlp_pack_elems() { return (size_t)(4 + 8 + 8) * 64 * 8; }
 int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
                const double* jbn, const double* delta, double* nut, double c4, double* vis) {
   const long long N = ctx->d.N();
-  __bf16* A1 = reinterpret_cast<__bf16*>(ctx->mlp_pack);
-  __bf16* A2 = A1 + 4 * 64 * 8;
-  __bf16* A3 = A2 + 8 * 64 * 8;
+  __bf16* A = reinterpret_cast<__bf16*>(ctx->mlp_pack);
   EMX_TIMED(ctx, "mlp_pack", 0.0,
             hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, ctx->stream, params, norm,
-                               A1, A2, A3));
-  const long long tiles = (N + 15) / 16;
-  long long blocks = (tiles + 3) / 4;
-  if (blocks > 8192) blocks = 8192;
-  // reads dvel (9), jbn (3), delta; writes nut
-  EMX_TIMED(ctx, "mlp_infer", 14.0 * 8.0 * N,
+                               A + MLP_PACK_A1, A + MLP_PACK_A2, A + MLP_PACK_A3));
+  static const int resident = resident_blocks(k_mlp_infer, 0);
+  const long long groups = (N + 63) / 64;
+  long long blocks = (groups + 3) / 4;
+  if (blocks > resident) blocks = resident;
+  // reads dvel (9), jbn (3), delta; writes nut (and vis)
+  EMX_TIMED(ctx, "mlp_infer", (vis ? 15.0 : 14.0) * 8.0 * N,
             hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                               dvel, jbn, delta, N, params, norm, A1, A2, A3, nut, c4, vis));
+                               dvel, jbn, delta, N, params, norm, A, nut, c4, vis));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -1,0 +1,184 @@
+// Inference of the learned eddy-viscosity closure on the matrix cores: the
+// 10 -> 64 -> 64 -> 64 -> 1 network (SiLU hidden, softplus output) for 64
+// grid points per wavefront, one point per lane.
+//
+// Each hidden layer is computed transposed, H^T[64 neurons x 16 points] =
+// W[64 x K] X^T[K x 16], with v_mfma_f32_16x16x32_bf16 (BF16 operands,
+// FP32 accumulation): four 16-neuron tiles (mt) per layer and, per
+// wavefront, four 16-point tiles (t) side by side, whose sixteen
+// independent products per layer keep the matrix core busy while the
+// activations of the previous tile are formed.  The 16x16 result a lane
+// holds (neurons 16mt + 4g + i of point n, g = lane/16, n = lane%16) is
+// already that lane's B operand of the next layer once the K order of the
+// next layer's weights is permuted to match (perm_in): hidden activations
+// never leave the lane.  The features go in through a wave-private LDS
+// transpose (10 x 64 floats), and the output reduction leaves lane l with
+// the value of point l of the group.
+#pragma once
+#include "compact.hpp"
+
+namespace emx {
+
+constexpr int MLP_IN = 10, MLP_H = 64;
+constexpr int MW1 = 0, MB1 = MW1 + MLP_H * MLP_IN, MW2 = MB1 + MLP_H,
+              MB2 = MW2 + MLP_H * MLP_H, MW3 = MB2 + MLP_H, MB3 = MW3 + MLP_H * MLP_H,
+              MW4 = MB3 + MLP_H, MB4 = MW4 + MLP_H, MNP = MB4 + 1;
+static_assert(MNP == EMX_MLP_NPARAM, "parameter count");
+
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+
+// Packed A operands (k_mlp_pack): A1[mt][lane][8] (layer 1, features
+// padded to K = 32), A2[mt][s][lane][8], A3[mt][s][lane][8]; lane l holds
+// row m = l%16 (neuron 16mt+m), K = 8(l/16) + j.
+constexpr int MLP_PACK_A1 = 0, MLP_PACK_A2 = 4 * 64 * 8, MLP_PACK_A3 = MLP_PACK_A2 + 8 * 64 * 8,
+              MLP_PACK_ELEMS = MLP_PACK_A3 + 8 * 64 * 8;
+
+// Input neuron of K index k = 8g + j (lane group g, element j) in K-step s
+// of layers 2 and 3: the lane's own result elements of output tiles 2s and
+// 2s+1 (neurons 16t + 4g + i).
+__host__ __device__ __forceinline__ int perm_in(int s, int g, int j) {
+  return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+
+// SiLU with the hardware reciprocal (1 ulp; BF16 operands carry 8 bits).
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float softplus_fast(float z) {
+  return z > 20.0f ? z : log1pf(__expf(z));
+}
+
+// One lane's share of the weights: its A operands of the three hidden
+// layers, kept in registers for the kernel's lifetime.
+struct MlpLaneWeights {
+  bf16x8 a1[4], a2[4][2], a3[4][2];
+};
+
+__device__ __forceinline__ void mlp_load_weights(MlpLaneWeights& W, const __bf16* A, int l) {
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) W.a1[mt][j] = A[MLP_PACK_A1 + (mt * 64 + l) * 8 + j];
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        W.a2[mt][s][j] = A[MLP_PACK_A2 + ((mt * 2 + s) * 64 + l) * 8 + j];
+        W.a3[mt][s][j] = A[MLP_PACK_A3 + ((mt * 2 + s) * 64 + l) * 8 + j];
+      }
+  }
+}
+
+// LDS copy of the vectors of the network, shared by a workgroup:
+// b1[64] b2[64] b3[64] w4[64] (P's layout), then b4 and out_scale.
+constexpr int MLP_VEC = 4 * MLP_H + 2;
+__device__ __forceinline__ void mlp_stage_vectors(float* vec, const float* P, const float* norm) {
+  for (int t = threadIdx.x; t < MLP_VEC; t += blockDim.x) {
+    float v;
+    if (t < MLP_H) v = P[MB1 + t];
+    else if (t < 2 * MLP_H) v = P[MB2 + t - MLP_H];
+    else if (t < 3 * MLP_H) v = P[MB3 + t - 2 * MLP_H];
+    else if (t < 4 * MLP_H) v = P[MW4 + t - 3 * MLP_H];
+    else if (t == 4 * MLP_H) v = P[MB4];
+    else v = norm[2 * MLP_IN];
+    vec[t] = v;
+  }
+}
+
+// Bias (or w4) of neurons 16t + 4g + 0..3 of vector row r.
+__device__ __forceinline__ f32x4 mlp_vec4(const float* vec, int r, int t, int g) {
+  return *reinterpret_cast<const f32x4*>(vec + r * MLP_H + 16 * t + 4 * g);
+}
+
+// nu_t (out_scale * softplus) of the lane's own point of a 64-point group.
+// f: the lane's normalised features; xs: this wave's LDS [MLP_IN][64];
+// vec: the staged vectors.  Every lane of the wave must call it.
+__device__ __forceinline__ float mlp_group64(const MlpLaneWeights& W, const float f[MLP_IN],
+                                             float* xs, const float* vec, int l) {
+  const int g = l >> 4, n = l & 15;
+#pragma unroll
+  for (int k = 0; k < MLP_IN; k++) xs[k * 64 + l] = f[k];
+  wave_sync();
+  float h[4][4][4];  // [point tile t][neuron tile mt][i]
+  // layer 1: B operand of tile t = features 8g..8g+7 of point 16t+n (K
+  // padded with zeros to 32)
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    bf16x8 x;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int k = 8 * g + j;
+      const float v = xs[(k < MLP_IN ? k : MLP_IN - 1) * 64 + 16 * t + n];
+      x[j] = (__bf16)(k < MLP_IN ? v : 0.0f);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a1[mt], x, acc, 0, 0, 0);
+      const f32x4 b = mlp_vec4(vec, 0, mt, g);
+#pragma unroll
+      for (int i = 0; i < 4; i++) h[t][mt][i] = silu_fast(acc[i] + b[i]);
+    }
+  }
+  wave_sync();  // xs is rewritten by the next group
+  // layers 2 and 3
+#pragma unroll
+  for (int layer = 0; layer < 2; layer++) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      bf16x8 bop[2];
+#pragma unroll
+      for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          bop[s][j] = (__bf16)(j < 4 ? h[t][2 * s][j] : h[t][2 * s + 1][j - 4]);
+#pragma unroll
+      for (int mt = 0; mt < 4; mt++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer == 0 ? W.a2[mt][s] : W.a3[mt][s],
+                                                          bop[s], acc, 0, 0, 0);
+        const f32x4 b = mlp_vec4(vec, 1 + layer, mt, g);
+#pragma unroll
+        for (int i = 0; i < 4; i++) h[t][mt][i] = silu_fast(acc[i] + b[i]);
+      }
+    }
+  }
+  // output neuron: FP32 dot product over the 64 neurons of each point, 16
+  // per lane, summed over the four lane groups; lane l keeps point l
+  float z[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    float s = 0.0f;
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+      const f32x4 w = mlp_vec4(vec, 3, mt, g);
+#pragma unroll
+      for (int i = 0; i < 4; i++) s += w[i] * h[t][mt][i];
+    }
+    s += __shfl_xor(s, 16);
+    s += __shfl_xor(s, 32);
+    z[t] = s;
+  }
+  const float zs = g == 0 ? z[0] : (g == 1 ? z[1] : (g == 2 ? z[2] : z[3]));
+  return vec[4 * MLP_H + 1] * softplus_fast(zs + vec[4 * MLP_H]);
+}
+
+// The ten normalised features of point p: du_a/dx_b = jbn(1+4b) *
+// dvel(3b+a) at index 3a+b, then delta; x = (f - shift) * scale.
+__device__ __forceinline__ void mlp_features(const double* dvel, const double* jbn,
+                                             const double* delta, long long N, long long p,
+                                             const float* norm, float x[MLP_IN]) {
+  const double m[3] = {jbn[p], jbn[4 * N + p], jbn[8 * N + p]};
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+      x[3 * a + b] = (float)((m[b] * dvel[(3 * b + a) * N + p] - norm[3 * a + b]) *
+                             norm[MLP_IN + 3 * a + b]);
+  x[9] = (float)((delta[p] - norm[9]) * norm[MLP_IN + 9]);
+}
+
+}  // namespace emx

@@ -7,9 +7,10 @@
   * BF16 matrix-core inference against the oracle's FP64 forward on the same
     features: 3e-2 normwise (BF16 operands carry 8 significant bits);
   * training by the library's own kernels lowers the loss, and the committed
-    trained network is closer to dynamic Smagorinsky than the
+    trained network is closer to dynamic Smagorinsky than the static
+    Smagorinsky model (one least-squares constant) and the
     structure-function sgdm on held-out snapshots (what the network gains
-    over the model without it).
+    over the models without it).
 """
 import os
 
@@ -119,10 +120,19 @@ def test_training_lowers_loss():
 @pytest.mark.skipif(not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "emix3d_amd",
                                                     "closure_weights.npz")),
                     reason="no trained closure weights in the tree")
-def test_trained_closure_beats_structure_function_model():
-    from emix3d_amd.closure import LearnedClosure, SnapshotFamily, evaluate
+def test_trained_closure_beats_static_smagorinsky_and_sgdm():
+    """Held-out snapshots (sine family seeds 201+, turbulent family seeds
+    1101+): the network's nu_t and viscous flux divergence are closer to
+    dynamic Smagorinsky than the static Smagorinsky model with the constant
+    fitted by least squares to the same training targets, and than the
+    structure-function sgdm."""
+    from emix3d_amd.closure import LearnedClosure, SnapshotFamily, StaticSmagorinsky, evaluate
     cl = LearnedClosure.load()
+    assert cl.static_c is not None and cl.static_c > 0.0
+    static = StaticSmagorinsky(cl.static_c)
     for kind, seeds in (("sines", range(201, 203)), ("turbulent", range(1101, 1103))):
-        ev = evaluate(SnapshotFamily(128, kind=kind), cl, seeds)
+        ev = evaluate(SnapshotFamily(128, kind=kind), cl, seeds, static)
         print(f"  held-out {kind} relative L2 vs dynamic Smagorinsky:", ev)
-        assert ev["mlp_nut"] < ev["sgdm_nut"], kind
+        for q in ("nut", "rsv"):
+            assert ev[f"mlp_{q}"] < ev[f"static_{q}"], (kind, q)
+            assert ev[f"mlp_{q}"] < ev[f"sgdm_{q}"], (kind, q)
