@@ -347,10 +347,15 @@ __device__ __forceinline__ void stencil_right(int q, int n, const double bd[6], 
 // given phi_int (the last stage), only the interior points, straight into
 // the unpadded phi (the values the reference's cuSwapFromGhost copies
 // back).  vel is the interior velocity (3 components of
-// (nx, ny, nz)): the padded velocity the reference builds is its mirror
-// image, so a padded point reads the interior point at its mirrored
-// coordinates instead of a padded copy.  The metrics (jbn, 11 components
-// of the interior grid; 1, 5, 9 used) are read the same way.
+// (nx, ny, nz)).  cuAdvectLsJB takes an already padded velocity from a
+// caller the reference does not have, so its ghost values are this
+// library's choice: the even mirror image of the interior for every
+// component (as phi is padded), read at the mirrored coordinates of the
+// interior point instead of from a padded copy.  Near the faces the
+// advected phi therefore follows that choice (an odd reflection of the
+// wall-normal component would be as plausible for a wall); the reference
+// does not pin it.  The metrics (jbn, 11 components of the interior grid;
+// 1, 5, 9 used) are read the same way.
 __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                                                  const double* P, const double* vel,
                                                  const double* jbn, PDims g, double iX,

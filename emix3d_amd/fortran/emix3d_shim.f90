@@ -79,16 +79,20 @@ SUBROUTINE viscosidad_t()
   use emx_capi
   IMPLICIT NONE
   integer(c_int) :: rc
-  if (.not. allocated(emx_sgd)) then
-    allocate(emx_sgd(nx, ny, nz, 7))
-    emx_sgd(:,:,:,1) = dxyzsgd
-    emx_sgd(:,:,:,2) = dxpsgd
-    emx_sgd(:,:,:,3) = dxmsgd
-    emx_sgd(:,:,:,4) = dypsgd
-    emx_sgd(:,:,:,5) = dymsgd
-    emx_sgd(:,:,:,6) = dzpsgd
-    emx_sgd(:,:,:,7) = dzmsgd
+  ! packed on every call: INISGDM may have run again (new grid or sizes)
+  ! since the last one, and the copy is small next to the transfer
+  if (allocated(emx_sgd)) then
+    if (size(emx_sgd, 1) /= nx .or. size(emx_sgd, 2) /= ny .or. &
+        size(emx_sgd, 3) /= nz) deallocate(emx_sgd)
   end if
+  if (.not. allocated(emx_sgd)) allocate(emx_sgd(nx, ny, nz, 7))
+  emx_sgd(:,:,:,1) = dxyzsgd
+  emx_sgd(:,:,:,2) = dxpsgd
+  emx_sgd(:,:,:,3) = dxmsgd
+  emx_sgd(:,:,:,4) = dypsgd
+  emx_sgd(:,:,:,5) = dymsgd
+  emx_sgd(:,:,:,6) = dzpsgd
+  emx_sgd(:,:,:,7) = dzmsgd
   rc = emx_host_sgdm(u, dvel, jbn, emx_sgd, c4, amut, vis, vis5, nx, ny, nz)
   if (rc /= 0) call emx_fail('viscosidad_t', rc)
 END SUBROUTINE viscosidad_t
