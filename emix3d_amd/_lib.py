@@ -41,6 +41,7 @@ SIGNATURES = {
     "emx_flux_div": ([_P, _I, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_flux_div_all": ([_P, _P, _P, _P, _P, _P, _P, _D, _D, _D, _P, _P], _I),
     "emx_dynsmag": ([_P, _P, _P, _P, _P, _P, _D, _P], _I),
+    "emx_dynsmag_noalloc": ([_I, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P], _I),
     "emx_rk_update": ([_P, _I, _P, _P, _P, _P, _P, _D], _I),
     "emx_frontera_val": ([_P, _P], _I),
     "emx_io_write_field": ([ctypes.c_char_p, _I, _I, _I, _I, _P, _P, _P], _I),

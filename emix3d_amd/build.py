@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libemix3d_hip.so")
 
 SOURCES = ["lines.hip", "ops_deriv.hip", "ops_flux.hip", "ops_sgs.hip", "ops_adj.hip", "closure.hip",
            "levelset.hip", "batch.hip", "rkglue.hip", "filter.hip", "fortran_io.hip",
-           "capi.hip"]
+           "noalloc.hip", "capi.hip"]
 
 # -ffp-contract=off keeps the FP64 operation order of the reference (no
 # fused multiply-adds), so GPU results sit within a few ulps of the oracle.

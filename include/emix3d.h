@@ -182,6 +182,19 @@ int emx_flux_div_all(emx_ctx* ctx, const double* um, const double* u,
 int emx_dynsmag(emx_ctx* ctx, const double* u, const double* dvel, const double* jbn,
                 const double* delta, double* nut, double c4, double* vis);
 
+/* emx_dynsmag without a context and without any device allocation: no
+ * counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its two scratch fields live in a static device array of
+ * 2 * 2^21 doubles (32 MiB per device, held until the process ends), so
+ * grids of at most 2^21 points (128^3): larger ones give EMX_ERR_SIZE
+ * before any launch.  Same kernels and results as emx_dynsmag, launched on
+ * stream (a hipStream_t; NULL = default stream).  Every call on one device
+ * shares that device's array: calls that overlap on a device race. */
+int emx_dynsmag_noalloc(int nx, int ny, int nz, const double* u, const double* dvel,
+                        const double* jbn, const double* delta, double* nut, double c4,
+                        double* vis, void* stream);
+
 /* ---- batched multi-grid stage ----------------------------------------------
  * No counterpart in the reference.  Many independent grids ("boxes") of
  * uneven sizes advanced together: per stage the gradient fused with the
