@@ -151,6 +151,19 @@ void free_batch(emx_batch* b) {
   delete b;
 }
 
+// Orders the batch's stream after everything queued so far on the caller's
+// stream, then waits until the batch's stream is idle.  The descriptor
+// tables are rewritten with synchronous copies on the null stream, which
+// does not wait for the batch's non-blocking stream: no kernel of the batch
+// may be reading them, and the caller's inputs must be complete before a
+// pass of the autotuner reads them.
+int quiesce(emx_batch* b) {
+  EMX_TRY_HIP(hipEventRecord(b->ev_in, b->user_stream));
+  EMX_TRY_HIP(hipStreamWaitEvent(b->stream, b->ev_in, 0));
+  EMX_TRY_HIP(hipStreamSynchronize(b->stream));
+  return EMX_OK;
+}
+
 int box_n(const Dims& d, int dir) { return dir == 0 ? d.nx : (dir == 1 ? d.ny : d.nz); }
 
 bool shape_fits(const emx_batch* b, int dir, BatchShape s) {
@@ -271,6 +284,7 @@ int commit(emx_batch* b) {
       set_error("batch: box %d has no bound data", i);
       return EMX_ERR_ARG;
     }
+  EMX_TRY(quiesce(b));
   for (int d = 0; d < 3; d++) EMX_TRY(plan_lines(b, d, b->shape[d]));
   std::vector<int> bp(b->nbox + 1, 0);
   b->ls_h.resize(b->nbox);
@@ -451,6 +465,7 @@ int emx_batch_tune(emx_batch* b, int reps) {
   }
   if (reps < 1) reps = 5;
   if (b->dirty) EMX_TRY(emx::commit(b));
+  EMX_TRY(emx::quiesce(b));
   hipEvent_t e0, e1;
   EMX_TRY_HIP(hipEventCreate(&e0));
   EMX_TRY_HIP(hipEventCreate(&e1));
@@ -460,7 +475,8 @@ int emx_batch_tune(emx_batch* b, int reps) {
     emx::BatchShape pick = b->shape[d];
     for (const auto& s : emx::kBatchShapes) {
       if (!emx::shape_fits(b, d, s)) continue;
-      rc = emx::plan_lines(b, d, s);
+      rc = emx::quiesce(b);  // the previous shape's launches read the tables
+      if (rc == EMX_OK) rc = emx::plan_lines(b, d, s);
       if (rc != EMX_OK) break;
       // passes 0..d so that the accumulator a pass reads is written
       for (int w = 0; w < 2 && rc == EMX_OK; w++)
@@ -481,10 +497,14 @@ int emx_batch_tune(emx_batch* b, int reps) {
         pick = s;
       }
     }
+    if (rc == EMX_OK) rc = emx::quiesce(b);
     if (rc == EMX_OK) rc = emx::plan_lines(b, d, pick);
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  // the timed passes wrote dvel and div: the caller's stream sees them after
+  if (rc == EMX_OK && hipEventRecord(b->ev_out, b->stream) == hipSuccess)
+    (void)hipStreamWaitEvent(b->user_stream, b->ev_out, 0);
   return rc;
 }
 

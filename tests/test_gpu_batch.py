@@ -178,3 +178,27 @@ def test_single_grid_grad_div(oracle):
     rdv, rdiv, _ = reference(oracle, z)
     report("grad_div dvel", normwise_rel(host(dvel), rdv))
     report("grad_div div", normwise_rel(host(div), rdiv))
+
+
+def test_rebind_and_replan_without_host_sync(oracle):
+    """stage, then bind one box with a new time step and change a line
+    shape, then stage again, with no host synchronisation in between: the
+    second stage runs on the new descriptors, which are uploaded only once
+    the first stage has finished, and every box matches two oracle steps."""
+    b, boxes = make_batch(DIMS[:3])
+    reset(boxes)
+    b.set_plan(1, 16, 4)
+    b.stage(graph=True)          # queued, not waited for
+    z = boxes[1]
+    dt2 = 0.5 * z["dt"]
+    b.bind(1, z["u"], z["jbn"], z["phi"], z["dvel"], z["div"], z["ih"], z["dxyz"], dt2)
+    b.set_plan(1, 32, 8)
+    b.stage(graph=True)
+    for i, z in enumerate(boxes):
+        dvel, div, phi1 = reference(oracle, z)
+        dt = dt2 if i == 1 else z["dt"]
+        phi2 = oracle.ls_advect_weno5(phi1, z["u"].cpu().numpy(), z["jbn"].cpu().numpy(),
+                                      z["dxyz"], dt)
+        report(f"box{i} dvel", normwise_rel(host(z["dvel"]), dvel))
+        report(f"box{i} div", normwise_rel(host(z["div"]), div))
+        report(f"box{i} phi after two stages", normwise_rel(host(z["phi"]), phi2))
