@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-glue", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only copy a 256 MiB buffer 5 times (the reference kernel for "
                          "scaling hardware byte counters) and exit")
@@ -109,20 +111,20 @@ def main():
         step()
     torch.cuda.synchronize()
     breakdown = kernel_breakdown(op.timing_end(), nb)
-    top = breakdown[0]
+    top = dominant_label(breakdown)
     # timed region: events around the launches of the dominant kernel only
-    op.timing_begin(label=top["label"])
+    op.timing_begin(label=top)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    live = [k for k in op.timing_end() if k["label"] == top["label"] and k["dir"] == top["dir"]]
+    live = [k for k in op.timing_end() if k["label"] == top]
     pts = float(n) ** 3
     value = pts / dt
     log(f"bench: {dt * 1e3:.3f} ms per stage, {value / 1e9:.3f} Gpts/s")
-    roof = kernel_roofline(top, live[0], breakdown)
+    roof = kernel_roofline(top, live, breakdown)
     for b in breakdown:
         log(f"bench:   {b['kernel']:44s} {b['us_per_launch']:8.1f} us x{b['launches_per_step']}"
             f"  {b['GBs']:6.0f} GB/s")
@@ -132,13 +134,24 @@ def main():
     # outputs dvel(9) dtemp(3) amut vis rs(5) rsv(5) = 24 fields of 8 B.
     stage_bytes_pt = (22 + 24) * 8
     stage_gbs = stage_bytes_pt * pts / dt / 1e9
+    # the same stage counted per drop-in entry point with the flux
+    # divergence fused per equation: gradient 128 (u, v, w, T in; 12 out),
+    # sgdm + viscosidad 168, flujos + divergencia of the five equations 600
+    fused_form_pt = 128 + 168 + 600
+    fused_form_gbs = fused_form_pt * pts / dt / 1e9
 
     learned = None
     if not args.no_learned:
-        learned = learned_section(op, f, jbn, sgd, c4, work, args.steps, args.warmup)
+        learned = learned_section(op, f, jbn, sgd, c4, work, dt, args.steps, args.warmup)
     batched = None
     if not args.no_batched:
         batched = batched_section(args.steps, args.warmup)
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_section(args.steps, args.warmup)
+    small = None
+    if not args.no_small:
+        small = small_grid_section(args.steps, args.warmup)
     glue = None
     if not args.no_glue:
         glue = glue_section(op, f, jbn, sgd, c4, work, dt, args.steps, args.warmup)
@@ -167,13 +180,17 @@ def main():
                    "velocity": "solenoidal k^-5/3 (k<=32) + 6-blade radial jet",
                    "stage_algorithmic_bytes_per_pt": stage_bytes_pt,
                    "stage_algorithmic_GBs": stage_gbs,
-                   "stage_frac_of_hbm_peak": stage_gbs / HBM_PEAK_GBS},
+                   "stage_frac_of_hbm_peak": stage_gbs / HBM_PEAK_GBS,
+                   "stage_fused_form_bytes_per_pt": fused_form_pt,
+                   "stage_fused_form_frac_of_hbm_peak": fused_form_gbs / HBM_PEAK_GBS},
         "roofline": roof,
         "kernels": breakdown,
         "cpu_baseline": cpu,
         "parity": parity,
         "batched": batched,
         "learned": learned,
+        "adjoint": adjoint,
+        "small_grid": small,
         "substep_and_filter": glue,
     }
     print(json.dumps(out), flush=True)
@@ -203,18 +220,30 @@ def kernel_breakdown(kernels, steps):
     return breakdown
 
 
-def kernel_roofline(top, live, breakdown):
-    """Roofline of the dominant kernel: its launches timed with HIP events
-    on the library's stream inside the timed region (live)."""
-    avg_ms = live["ms"] / live["launches"]
-    b = live["bytes"] / live["launches"]
+def dominant_label(breakdown):
+    """The kernel label with the largest total time over all its launches
+    and directions (the line kernels of one label share their code)."""
+    tot = {}
+    for b in breakdown:
+        tot[b["label"]] = tot.get(b["label"], 0.0) + b["ms_per_step"]
+    return max(tot, key=tot.get)
+
+
+def kernel_roofline(label, live, breakdown):
+    """Roofline of the dominant kernel: every launch of its label timed with
+    HIP events on the library's stream inside the timed region (live)."""
+    launches = sum(k["launches"] for k in live)
+    ms = sum(k["ms"] for k in live)
+    b = sum(k["bytes"] for k in live) / launches
+    avg_ms = ms / launches
     gbs = b / (avg_ms * 1e-3) / 1e9
-    return {"kernel": top["kernel"], "bound": "hbm", "achieved": gbs,
+    share = (sum(x["ms_per_step"] for x in breakdown if x["label"] == label) /
+             sum(x["ms_per_step"] for x in breakdown))
+    return {"kernel": label, "bound": "hbm", "achieved": gbs,
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC.get(top["kernel"]),
+            "traffic": MEASURED_TRAFFIC.get(label),
             "algorithmic_bytes_per_launch": b, "avg_launch_ms": avg_ms,
-            "launches_timed": live["launches"],
-            "share_of_step": top["ms_per_step"] / sum(x["ms_per_step"] for x in breakdown)}
+            "launches_timed": launches, "share_of_step": share}
 
 
 def timed(fn, steps, warmup):
@@ -310,15 +339,18 @@ def glue_section(op, f, jbn, sgd, c4, work, stage_s, steps, warmup):
     return out
 
 
-def learned_section(op, f, jbn, sgd, c4, work, steps, warmup):
+def learned_section(op, f, jbn, sgd, c4, work, plain_s, steps, warmup):
     """The 256^3 stage with the eddy viscosity from the learned closure
     (BF16 matrix cores) against the same stage with dynamic Smagorinsky
-    computed directly: throughput of both, and the relative L2 error of
-    nu_t and of the viscous flux divergence rsv (equations 2..4) of the
-    learned form against the dynamic one.  The structure-function sgdm
-    (the reference's model) is scored against the same target."""
+    computed directly (the model the network learns) and against the plain
+    stage (structure-function sgdm, the reference's model, timed above):
+    throughput of each, the closure kernels alone, and the
+    relative L2 error of nu_t and of the viscous flux divergence rsv
+    (equations 2..4) against the dynamic model, for the network, the static
+    Smagorinsky model (one least-squares constant fitted to the training
+    targets) and sgdm."""
     import torch
-    from emix3d_amd.closure import WEIGHTS, LearnedClosure
+    from emix3d_amd.closure import WEIGHTS, LearnedClosure, StaticSmagorinsky
     if not os.path.exists(WEIGHTS):
         log("bench: no trained closure weights; learned section skipped")
         return None
@@ -344,20 +376,102 @@ def learned_section(op, f, jbn, sgd, c4, work, steps, warmup):
     rsv_dyn = rsv[1:4].clone()
     t_mlp = timed(lambda: stage("mlp"), steps, warmup)
     rsv_mlp = rsv[1:4].clone()
+    # the closures alone
+    op.timing_begin()
+    cl.nut(op, dv, jbn, delta, out=nut_m, c4=c4, vis=vis)
+    op.dynsmag(u, dv, jbn, delta, nut=nut_d, c4=c4, vis=vis)
+    torch.cuda.synchronize()
+    kern = {k["label"]: {"us": k["ms"] * 1e3, "GBs": k["bytes"] / (k["ms"] * 1e-3) / 1e9}
+            for k in op.timing_end() if k["label"] in ("mlp_infer", "dynsmag_lm", "dynsmag_nut")}
+    pts = float(delta.numel())
+    if "mlp_infer" in kern:
+        kern["mlp_infer"]["TFLOPs"] = 17792.0 * pts / (kern["mlp_infer"]["us"] * 1e-6) / 1e12
     amut, _, _ = op.viscosidad_t(u, dv, jbn, sgd, c4, want_vis5=False)
     op.flujos_divergencia_all(um, u, pres, (c4 + amut).contiguous(), jbn, dv, rs=rs, rsv=rsv)
     rsv_sgdm = rsv[1:4].clone()
-    pts = float(delta.numel())
     out = {"workload": "256^3 curvilinear stage (grad + closure + flux divergence of 5 eqs)",
+           "plain_sgdm_pts_per_s": pts / plain_s,
            "dynamic_smagorinsky_pts_per_s": pts / t_dyn,
            "learned_pts_per_s": pts / t_mlp,
-           "speedup": t_dyn / t_mlp,
+           "learned_vs_plain": plain_s / t_mlp,
+           "learned_vs_dynamic": t_dyn / t_mlp,
+           "closure_kernels": kern,
            "nut_rel_l2_vs_dynamic": rel_l2(nut_m, nut_d),
            "rsv_rel_l2_vs_dynamic": rel_l2(rsv_mlp, rsv_dyn),
            "sgdm_nut_rel_l2_vs_dynamic": rel_l2(amut, nut_d),
            "sgdm_rsv_rel_l2_vs_dynamic": rel_l2(rsv_sgdm, rsv_dyn),
            "inference": "BF16 MFMA (16x16x32), FP32 accumulation, FP64 stage"}
+    if cl.static_c is not None:
+        nut_s = StaticSmagorinsky(cl.static_c).nut(dv, jbn, delta)
+        op.flujos_divergencia_all(um, u, pres, (c4 + nut_s).contiguous(), jbn, dv, rs=rs, rsv=rsv)
+        out["static_smagorinsky_c"] = cl.static_c
+        out["static_nut_rel_l2_vs_dynamic"] = rel_l2(nut_s, nut_d)
+        out["static_rsv_rel_l2_vs_dynamic"] = rel_l2(rsv[1:4], rsv_dyn)
     log(f"bench: learned closure {out}")
+    return out
+
+
+def adjoint_section(steps, warmup):
+    """The adjoint of the composed operator (velocity gradient, fluxes of
+    the five equations, divergence) on a 128^3 tanh-stretched grid with the
+    sine-mode field: emx_flux_div_all_adj + emx_grad_adj of random
+    cotangents, points per second and the library's algorithmic bytes per
+    launch."""
+    import torch
+    from emix3d_amd import Operator, fields
+    n = 128
+    f = fields.stage_fields(n, n, n, 20261206, curvilinear=True, turbulent=False, device="cuda")
+    op = Operator(n, n, n, "c6c")
+    jbn = op.jacobeano(f["x"])
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    vis = 1e-3 + 1e-3 * torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    rb = torch.randn((5, n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    rvb = torch.randn((5, n, n, n), dtype=torch.float64, device="cuda", generator=gen)
+    dvel, _ = op.deriv_vel(f["u"])
+    outb = None
+
+    def adj():
+        nonlocal outb
+        outb = op.flujos_divergencia_all_adj(rb, rvb, f["um"], f["u"], vis, jbn, dvel, out=outb)
+        op.deriv_vel_adj(outb["dvel_bar"])
+
+    t = timed(adj, steps, warmup)
+    op.timing_begin()
+    adj()
+    torch.cuda.synchronize()
+    ks = op.timing_end()
+    ms = sum(k["ms"] for k in ks)
+    byt = sum(k["bytes"] for k in ks)
+    pts = float(n) ** 3
+    out = {"workload": "128^3 tanh-stretched adjoint of grad + fluxes + divergence (5 eqs)",
+           "pts_per_s": pts / t, "ms": t * 1e3,
+           "algorithmic_bytes_per_pt": byt / pts,
+           "GBs_algorithmic": byt / (ms * 1e-3) / 1e9,
+           "frac_of_hbm_peak": byt / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "kernels": {k["label"] + (f" dir{k['dir']}" if k["dir"] >= 0 else ""):
+                       {"launches": k["launches"], "us": k["ms"] * 1e3,
+                        "GBs": k["bytes"] / (k["ms"] * 1e-3) / 1e9} for k in ks}}
+    log(f"bench: adjoint {out}")
+    return out
+
+
+def small_grid_section(steps, warmup):
+    """The 64^3 uniform grid with 8 sine modes per axis: the C6F velocity
+    gradient fused with the velocity divergence (emx_grad_div).  One field
+    is 2 MiB, so the working set sits in the 256 MB last-level cache: points
+    per second only, no HBM roofline."""
+    import torch
+    from emix3d_amd import Operator, fields
+    n = 64
+    x = fields.grid_uniform(n, n, n, device="cuda")
+    u = fields.velocity_sines(x, 20261203).contiguous()
+    op = Operator(n, n, n, "c6f")
+    jbn = op.jacobeano(x)
+    dvel, div = op.grad_div(u, jbn=jbn)
+    t = timed(lambda: op.grad_div(u, jbn=jbn, dvel=dvel, div=div), max(steps, 50), warmup)
+    out = {"workload": "64^3 uniform, 8 sine modes per axis: C6F gradient + velocity divergence",
+           "pts_per_s": float(n) ** 3 / t, "us": t * 1e6}
+    log(f"bench: small grid {out}")
     return out
 
 

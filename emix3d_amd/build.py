@@ -27,6 +27,12 @@ HIPCC_FLAGS = [
 ]
 
 
+# Per-source additions.  closure.hip: the matrix cores' results in ordinary
+# VGPRs, so the inference kernel copies nothing out of AGPRs (not used for
+# ops_flux.hip, whose fused closure variant holds its spills in AGPRs).
+SOURCE_FLAGS = {"closure.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def hipcc():
     for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
         if cand and (os.path.isabs(cand) and os.path.exists(cand) or not os.path.isabs(cand)):
@@ -46,7 +52,8 @@ def build(verbose=False, jobs=None):
         src = os.path.join(CSRC, s)
         obj = os.path.join(objdir, s.replace(".hip", ".o"))
         objs.append(obj)
-        cmd = [cc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+        cmd = [cc, *HIPCC_FLAGS, *SOURCE_FLAGS.get(s, []), "-I", os.path.join(ROOT, "include"),
+               "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd))
         procs.append((s, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

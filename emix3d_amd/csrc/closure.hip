@@ -57,7 +57,9 @@ __global__ void k_mlp_pack(const float* P, const float* norm, __bf16* A1, __bf16
     const int o = 16 * mt + m;
     for (int j = 0; j < 8; j++) {
       const int k = 8 * g + j;
-      A1[(mt * 64 + l) * 8 + j] = (__bf16)(k < MLP_IN ? P[MW1 + o * MLP_IN + k] : 0.0f);
+      // layer 1 in the scaled form of mlp_group64: y = MLP_YS * z
+      A1[(mt * 64 + l) * 8 + j] =
+          (__bf16)(k < MLP_IN ? MLP_YS * P[MW1 + o * MLP_IN + k] : 0.0f);
       for (int s = 0; s < 2; s++) {
         const int in = perm_in(s, g, j);
         A2[((mt * 2 + s) * 64 + l) * 8 + j] = (__bf16)P[MW2 + o * MLP_H + in];
@@ -76,9 +78,9 @@ k_mlp_infer(const double* dvel, const double* jbn, const double* delta, long lon
             double* vis) {
   __shared__ __attribute__((aligned(16))) float vec[MLP_VEC + 2];
   __shared__ float xs[4][MLP_IN * 64];
+  __shared__ bf16x8 lw[MLP_LW];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
-  MlpLaneWeights W;
-  mlp_load_weights(W, A, l);
+  mlp_stage_weights(lw, A);
   mlp_stage_vectors(vec, P, norm);
   __syncthreads();
   const long long ngroup = (N + 63) / 64;
@@ -110,7 +112,7 @@ k_mlp_infer(const double* dvel, const double* jbn, const double* delta, long lon
         x[3 * a + b] = (float)((cur[9 + b] * cur[3 * b + a] - norm[3 * a + b]) *
                                norm[MLP_IN + 3 * a + b]);
     x[9] = (float)((cur[12] - norm[9]) * norm[MLP_IN + 9]);
-    const float nu = mlp_group64(W, x, xs[w], vec, l);
+    const float nu = mlp_group64(lw, x, xs[w], vec, l);
     const long long p = grp * 64 + l;
     if (p < N) {
       nut[p] = (double)nu;
@@ -302,13 +304,20 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, int n, floa
 
 size_t mlp_pack_elems() { return (size_t)(4 + 8 + 8) * 64 * 8; }
 
-int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
-               const double* jbn, const double* delta, double* nut, double c4, double* vis) {
-  const long long N = ctx->d.N();
+int op_mlp_pack(emx_ctx* ctx, const float* params, const float* norm) {
   __bf16* A = reinterpret_cast<__bf16*>(ctx->mlp_pack);
   EMX_TIMED(ctx, "mlp_pack", 0.0,
             hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, ctx->stream, params, norm,
                                A + MLP_PACK_A1, A + MLP_PACK_A2, A + MLP_PACK_A3));
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
+int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
+               const double* jbn, const double* delta, double* nut, double c4, double* vis) {
+  const long long N = ctx->d.N();
+  __bf16* A = reinterpret_cast<__bf16*>(ctx->mlp_pack);
+  EMX_TRY(op_mlp_pack(ctx, params, norm));
   static const int resident = resident_blocks(k_mlp_infer, 0);
   const long long groups = (N + 63) / 64;
   long long blocks = (groups + 3) / 4;

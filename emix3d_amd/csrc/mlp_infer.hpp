@@ -41,45 +41,42 @@ __host__ __device__ __forceinline__ int perm_in(int s, int g, int j) {
   return 32 * s + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
 }
 
-// SiLU with the hardware reciprocal (1 ulp; BF16 operands carry 8 bits).
-__device__ __forceinline__ float silu_fast(float z) {
-  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+// The hidden layers run in a scaled form that leaves one exponential, one
+// reciprocal, one add and one multiply per activation: with s = -log2(e),
+// each layer forms y = s z (its bias scaled by s, and layer 1's weights),
+// so that e^-z = 2^y, and keeps h~ = y / (1 + 2^y) = s silu(z) as the next
+// layer's input; the next layer's y' = s (W h + b) = W h~ + s b needs no
+// other change, and the output layer's weights are divided by s.  The
+// reciprocal is the hardware one (1 ulp; the BF16 operands carry 8 bits).
+constexpr float MLP_YS = -1.44269504088896340736f;
+__device__ __forceinline__ float silu_scaled(float y) {
+  return y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y));
 }
 __device__ __forceinline__ float softplus_fast(float z) {
   return z > 20.0f ? z : log1pf(__expf(z));
 }
 
-// One lane's share of the weights: its A operands of the three hidden
-// layers, kept in registers for the kernel's lifetime.
-struct MlpLaneWeights {
-  bf16x8 a1[4], a2[4][2], a3[4][2];
-};
-
-__device__ __forceinline__ void mlp_load_weights(MlpLaneWeights& W, const __bf16* A, int l) {
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++) {
-#pragma unroll
-    for (int j = 0; j < 8; j++) W.a1[mt][j] = A[MLP_PACK_A1 + (mt * 64 + l) * 8 + j];
-#pragma unroll
-    for (int s = 0; s < 2; s++)
-#pragma unroll
-      for (int j = 0; j < 8; j++) {
-        W.a2[mt][s][j] = A[MLP_PACK_A2 + ((mt * 2 + s) * 64 + l) * 8 + j];
-        W.a3[mt][s][j] = A[MLP_PACK_A3 + ((mt * 2 + s) * 64 + l) * 8 + j];
-      }
-  }
+// The packed A operands staged in LDS by a workgroup (20 KB, MLP_PACK_ELEMS
+// BF16 values in units of bf16x8: layer 1 at 0, layer 2 at 256, layer 3 at
+// 768).  A wavefront reads each operand once per layer for all four point
+// tiles, so the weights take no registers between layers.
+constexpr int MLP_LW = MLP_PACK_ELEMS / 8;
+__device__ __forceinline__ void mlp_stage_weights(bf16x8* lw, const __bf16* A) {
+  const bf16x8* src = reinterpret_cast<const bf16x8*>(A);
+  for (int t = threadIdx.x; t < MLP_LW; t += blockDim.x) lw[t] = src[t];
 }
 
-// LDS copy of the vectors of the network, shared by a workgroup:
-// b1[64] b2[64] b3[64] w4[64] (P's layout), then b4 and out_scale.
+// LDS copy of the vectors of the network, shared by a workgroup, in the
+// scaled form: s b1[64], s b2[64], s b3[64], w4[64] / s, then b4 and
+// out_scale.
 constexpr int MLP_VEC = 4 * MLP_H + 2;
 __device__ __forceinline__ void mlp_stage_vectors(float* vec, const float* P, const float* norm) {
   for (int t = threadIdx.x; t < MLP_VEC; t += blockDim.x) {
     float v;
-    if (t < MLP_H) v = P[MB1 + t];
-    else if (t < 2 * MLP_H) v = P[MB2 + t - MLP_H];
-    else if (t < 3 * MLP_H) v = P[MB3 + t - 2 * MLP_H];
-    else if (t < 4 * MLP_H) v = P[MW4 + t - 3 * MLP_H];
+    if (t < MLP_H) v = MLP_YS * P[MB1 + t];
+    else if (t < 2 * MLP_H) v = MLP_YS * P[MB2 + t - MLP_H];
+    else if (t < 3 * MLP_H) v = MLP_YS * P[MB3 + t - 2 * MLP_H];
+    else if (t < 4 * MLP_H) v = P[MW4 + t - 3 * MLP_H] / MLP_YS;
     else if (t == 4 * MLP_H) v = P[MB4];
     else v = norm[2 * MLP_IN];
     vec[t] = v;
@@ -92,15 +89,19 @@ __device__ __forceinline__ f32x4 mlp_vec4(const float* vec, int r, int t, int g)
 }
 
 // nu_t (out_scale * softplus) of the lane's own point of a 64-point group.
-// f: the lane's normalised features; xs: this wave's LDS [MLP_IN][64];
-// vec: the staged vectors.  Every lane of the wave must call it.
-__device__ __forceinline__ float mlp_group64(const MlpLaneWeights& W, const float f[MLP_IN],
+// lw: the staged weights; f: the lane's normalised features; xs: this
+// wave's LDS [MLP_IN][64]; vec: the staged vectors.  Every lane of the wave
+// must call it.
+__device__ __forceinline__ float mlp_group64(const bf16x8* lw, const float f[MLP_IN],
                                              float* xs, const float* vec, int l) {
   const int g = l >> 4, n = l & 15;
 #pragma unroll
   for (int k = 0; k < MLP_IN; k++) xs[k * 64 + l] = f[k];
   wave_sync();
   float h[4][4][4];  // [point tile t][neuron tile mt][i]
+  bf16x8 a1[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; mt++) a1[mt] = lw[mt * 64 + l];
   // layer 1: B operand of tile t = features 8g..8g+7 of point 16t+n (K
   // padded with zeros to 32)
 #pragma unroll
@@ -114,17 +115,22 @@ __device__ __forceinline__ float mlp_group64(const MlpLaneWeights& W, const floa
     }
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a1[mt], x, acc, 0, 0, 0);
-      const f32x4 b = mlp_vec4(vec, 0, mt, g);
+      // the (scaled) bias is the accumulator's initial value
+      f32x4 acc = mlp_vec4(vec, 0, mt, g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[mt], x, acc, 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < 4; i++) h[t][mt][i] = silu_fast(acc[i] + b[i]);
+      for (int i = 0; i < 4; i++) h[t][mt][i] = silu_scaled(acc[i]);
     }
   }
   wave_sync();  // xs is rewritten by the next group
   // layers 2 and 3
 #pragma unroll
   for (int layer = 0; layer < 2; layer++) {
+    bf16x8 a[4][2];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+      for (int s = 0; s < 2; s++) a[mt][s] = lw[256 + layer * 512 + (mt * 2 + s) * 64 + l];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
       bf16x8 bop[2];
@@ -135,14 +141,12 @@ __device__ __forceinline__ float mlp_group64(const MlpLaneWeights& W, const floa
           bop[s][j] = (__bf16)(j < 4 ? h[t][2 * s][j] : h[t][2 * s + 1][j - 4]);
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        f32x4 acc = mlp_vec4(vec, 1 + layer, mt, g);
 #pragma unroll
         for (int s = 0; s < 2; s++)
-          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(layer == 0 ? W.a2[mt][s] : W.a3[mt][s],
-                                                          bop[s], acc, 0, 0, 0);
-        const f32x4 b = mlp_vec4(vec, 1 + layer, mt, g);
+          acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][s], bop[s], acc, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; i++) h[t][mt][i] = silu_fast(acc[i] + b[i]);
+        for (int i = 0; i < 4; i++) h[t][mt][i] = silu_scaled(acc[i]);
       }
     }
   }
@@ -156,7 +160,7 @@ __device__ __forceinline__ float mlp_group64(const MlpLaneWeights& W, const floa
     for (int mt = 0; mt < 4; mt++) {
       const f32x4 w = mlp_vec4(vec, 3, mt, g);
 #pragma unroll
-      for (int i = 0; i < 4; i++) s += w[i] * h[t][mt][i];
+      for (int i = 0; i < 4; i++) s = __builtin_fmaf(w[i], h[t][mt][i], s);
     }
     s += __shfl_xor(s, 16);
     s += __shfl_xor(s, 32);

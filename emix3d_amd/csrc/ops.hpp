@@ -60,6 +60,10 @@ int launch_dynsmag(emx_ctx* ctx, hipStream_t stream, const Dims& d, const double
 size_t mlp_pack_elems();
 int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
                const double* jbn, const double* delta, double* nut, double c4, double* vis);
+// packs the network's BF16 operands into ctx->mlp_pack
+int op_mlp_pack(emx_ctx* ctx, const float* params, const float* norm);
+
+
 int op_mlp_gather(emx_ctx* ctx, const double* dvel, const double* jbn, const double* delta,
                   const long long* idx, int n, float* feat);
 

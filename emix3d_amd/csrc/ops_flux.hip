@@ -84,6 +84,14 @@ static int project_reads(int mask) {
 // the eight xi lines with the wavefront kind's solve and writes them as
 // the first-pass accumulators (ProbAccum pass 0).  The xi fluxes never go
 // to memory.
+//
+// (Forming the eddy viscosity in this pass instead of reading vis was
+// measured slower at 256^3.  The structure-function model: 2.7 ms against
+// 1.96 ms for the sgdm kernel and this pass, its stencil needing the
+// velocity and vorticity of the four neighbour lines along eta and zeta,
+// whose loads could not be kept in flight.  The learned closure: 2.1-2.9 ms
+// against 2.2 ms, its matrix-core network and SiLU arithmetic serialised
+// with the streaming at the one or two waves per SIMD its registers allow.)
 template <int S, int C>
 __global__ void __launch_bounds__(256)
 k_flux_project_xi(FluxIn F, LineTab T, long long nlines, int nx, int ny, int yz, int mat,
@@ -298,7 +306,10 @@ int op_flux_div_all(emx_ctx* ctx, const FluxPtrs& fp, const double ih[3],
   int pass = fused_xi ? 1 : 0, k = fused_xi ? 1 : 0;
   for (int dir = fused_xi ? 1 : 0; dir < 3; dir++) {
     if (!(mat & (1 << dir))) continue;
-    // slots (0,1) (2,3) (4,5) (6,7): outputs rs 1..5 then rsv 2..4
+    // slots (0,1) (2,3) (4,5) (6,7): outputs rs 1..5 then rsv 2..4.  (Four
+    // slots per launch, which reads 1/J once per four fluxes in the last
+    // pass, measured 1.6x slower per flux on the 32-lines-per-wave kind:
+    // twice the values per work-item.)
     for (int slot = 0; slot < 8; slot += 2) {
       double* o0 = slot < 5 ? rs + slot * N : rsv + (slot - 4) * N;
       double* o1 = slot + 1 < 5 ? rs + (slot + 1) * N : rsv + (slot - 3) * N;

@@ -225,7 +225,9 @@ int emx_batch_bind(emx_batch* b, int i, const double* u, const double* jbn, doub
 int emx_batch_set_plan(emx_batch* b, int dir, int lg, int C);
 int emx_batch_get_plan(const emx_batch* b, int dir, int* lg, int* C);
 /* Autotuner: times every built shape of each direction's line pass on the
- * bound data (reps launches each) and keeps the fastest. */
+ * bound data (reps launches each) and keeps the fastest.  It runs after the
+ * work queued on the batch's stream argument and overwrites the bound dvel
+ * and div (phi is untouched); later work on that stream sees its writes. */
 int emx_batch_tune(emx_batch* b, int reps);
 /* One stage for all boxes; use_graph != 0 captures the launch sequence on
  * the first call (after a bind or plan change) and replays it. */
