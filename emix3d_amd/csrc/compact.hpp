@@ -524,6 +524,25 @@ struct BatchLines {
   double ih;
 };
 
+// Workgroup local (of ngroup in its grid) -> line group g, row-major over
+// (group along the lines, outer index).  A grid's rows start anywhere in a
+// 128-byte cache line (its nx is any size), so neighbouring groups along a
+// row share a cache line at every point.  Workgroups go to the eight XCDs
+// round robin, each XCD with its own L2: within each set of 8K workgroups
+// (K = 16, fewer in the grid's last set), XCD x takes the K consecutive
+// groups xK .. xK+K-1, in order, so that all but one of their shared lines
+// are fetched once into that XCD's L2 and read there by the neighbour
+// dispatched next.  A bijection on [0, ngroup): the last local % 8 ids of
+// the grid keep their own group.
+__device__ __forceinline__ int xcd_run_group(int local, int ngroup) {
+  constexpr int K = 16;
+  const int base = local & ~(8 * K - 1);
+  const int k = min(K, (ngroup - base) >> 3);  // groups per XCD in this set
+  const int r = local - base;
+  if (r >= 8 * k) return local;
+  return base + (r & 7) * k + (r >> 3);
+}
+
 template <int S, int C, int LG, class Prob>
 __global__ void __launch_bounds__(1024)
 k_lines_batched(const BatchLines<Prob>* grids, const int* blk_start, int ngrid) {
@@ -535,9 +554,9 @@ k_lines_batched(const BatchLines<Prob>* grids, const int* blk_start, int ngrid) 
     else hi = mid;
   }
   const BatchLines<Prob>& G = grids[lo];
-  const int local = b - blk_start[lo];
-  lines_strided_body<S, C, LG>(G.prob, G.T, G.nlanes, G.si, G.so, G.sl, G.ih,
-                               line_group<LG>(local % G.gx, G.gx), local / G.gx);
+  const int g = xcd_run_group(b - blk_start[lo], blk_start[lo + 1] - blk_start[lo]);
+  lines_strided_body<S, C, LG>(G.prob, G.T, G.nlanes, G.si, G.so, G.sl, G.ih, g % G.gx,
+                               g / G.gx);
 }
 
 // The wavefront kind for many grids in one launch (xi lines, contiguous in
