@@ -54,6 +54,7 @@ SIGNATURES = {
     "emx_io_write_scalar": ([ctypes.c_char_p, _I, _I, _I, _I, _P], _I),
     "emx_io_read_scalar": ([ctypes.c_char_p, _I, _I, _I, _I, _P], _I),
     "emx_ls_reinit": ([_P, _P, _P, _P, _D, _D, _D, _D], _I),
+    "emx_heviside": ([_P, _P, _D, _P], _I),
     "emx_varc_to_varnc": ([_P, _P, _D, _P, _P, _P, _P], _I),
     "emx_varnc_to_varc": ([_P, _P, _P, _P, _D, _P, _P], _I),
     "emx_rk_clip": ([_P, _P, _P, _D, _P, _P, _P, _P], _I),

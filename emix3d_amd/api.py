@@ -436,6 +436,14 @@ class Operator:
                   self._check(dmin), *map(float, dxyz), float(deltamin))
         return phi
 
+    def heviside(self, phi, deltamax, hevi=None):
+        """heviside: the smoothed Heaviside of phi over |phi| <= 1.5
+        deltamax (solver/advectLS.f90:29-59); hevi may be phi (in place)."""
+        hevi = self._empty() if hevi is None else hevi
+        self._stream()
+        _lib.call("emx_heviside", self._ctx, self._check(phi), float(deltamax), self._check(hevi))
+        return hevi
+
     def advect_weno5(self, phi, vel, jbn, dxyz, dt):
         """One SSP-RK3 WENO5 advection step of phi (in place)."""
         self._stream()

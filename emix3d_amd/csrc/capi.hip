@@ -1,5 +1,6 @@
 // extern "C" boundary of libemix3d_hip.so (declared in include/emix3d.h).
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -454,6 +455,17 @@ int emx_rk_update(emx_ctx* ctx, int sub, double* um1, const double* um0, const d
     return EMX_ERR_ARG;
   }
   return op_rk_update(ctx, sub, um1, um0, um, rs, rsv, dt);
+}
+
+int emx_heviside(emx_ctx* ctx, const double* phi, double deltamax, double* hevi) {
+  EMX_TRY(check_ctx(ctx));
+  CHECK_PTR(phi);
+  CHECK_PTR(hevi);
+  if (!(deltamax > 0.0 && std::isfinite(deltamax))) {
+    set_error("emx_heviside: deltamax must be positive and finite (got %g)", deltamax);
+    return EMX_ERR_ARG;
+  }
+  return op_ls_heviside(ctx, phi, deltamax, hevi);
 }
 
 int emx_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmin, double dx,

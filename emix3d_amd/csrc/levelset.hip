@@ -727,4 +727,31 @@ int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
   return EMX_OK;
 }
 
+namespace {
+
+// heviside (solver/advectLS.f90:29-59): the smoothed Heaviside of phi over
+// the band |phi| <= eps, in the reference's order of operations.  A point
+// that none of the three cases takes (NaN phi) keeps its hevi, as there.
+__global__ void k_heviside(const double* phi, long long N, double eps, double* hevi) {
+  const double pi = 3.14159265358979323846;  // 2 acos(0) in double
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < N;
+       p += (long long)gridDim.x * blockDim.x) {
+    const double f = phi[p];
+    if (f < -eps) hevi[p] = 0.0;
+    if (f >= -eps && f <= eps) hevi[p] = 0.5 + f / (2.0 * eps) + sin(pi * f / eps) / (2.0 * pi);
+    if (f > eps) hevi[p] = 1.0;
+  }
+}
+
+}  // namespace
+
+int op_ls_heviside(emx_ctx* ctx, const double* phi, double deltamax, double* hevi) {
+  const long long N = ctx->d.N();
+  EMX_TIMED(ctx, "ls_heviside", 16.0 * N,
+            hipLaunchKernelGGL(k_heviside, dim3(blocks_for(N)), dim3(256), 0, ctx->stream, phi, N,
+                               1.5 * deltamax, hevi));
+  EMX_TRY_HIP(hipGetLastError());
+  return EMX_OK;
+}
+
 }  // namespace emx

@@ -105,5 +105,6 @@ int op_ls_extrapolate(emx_ctx* ctx, double* pres, const double* jbn,
 int op_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
                        const double* jbn, double dx, double dy, double dz,
                        double dt);
+int op_ls_heviside(emx_ctx* ctx, const double* phi, double deltamax, double* hevi);
 
 }  // namespace emx

@@ -376,6 +376,14 @@ int emx_ls_advect_weno5(emx_ctx* ctx, double* phi, const double* vel,
 int emx_ls_reinit(emx_ctx* ctx, double* phi, const double* jbn, const double* dmin, double dx,
                   double dy, double dz, double deltamin);
 
+/* Smoothed Heaviside of the level set, heviside (solver/advectLS.f90:29-59):
+ * with eps = 1.5 deltamax, hevi = 0 where phi < -eps, 1 where phi > eps,
+ * and 0.5 + phi/(2 eps) + sin(pi phi/eps)/(2 pi) in between; a NaN phi
+ * leaves its hevi unchanged, as in the reference.  deltamax is INISGDM's
+ * largest cell size (emx_sgs_geometry returns it); it must be positive and
+ * finite (EMX_ERR_ARG).  phi == hevi (in place) is allowed. */
+int emx_heviside(emx_ctx* ctx, const double* phi, double deltamax, double* hevi);
+
 /* ---- the reference's on-disk formats (host memory, no device use) -------
  * Fortran unformatted sequential files: each record a 4-byte length, the
  * data, the length again; 4-byte integers, 8-byte reals, arrays

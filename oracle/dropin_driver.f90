@@ -8,11 +8,15 @@
 ! (solver/rungek_ico.f90:79-95 and :125) through those unmodified call
 ! sites and writes the results for the test to compare with the oracle.
 !
-! Usage: emx_dropin <input file> <output file>
+! Usage: emx_dropin <input file> <output file> [heviside]
 ! Input (stream, native endianness): nx, ny, nz (int32); deltamin, c4
 ! (float64); then u(3N), temp(N), um(5N), pres(N), jbn(11N), sgd(7N),
 ! sobject(N).  Output: dvel(9N), dtemp(3N), amut(N), vis(N), vis5(N),
 ! rs(5N), rsv(5N), pres(N) after advectLevelSet.
+! With a third argument "heviside" it runs the reference's heviside
+! (solver/advectLS.f90:29-59, CPU Fortran, no device use) instead: input
+! nx, ny, nz (int32), deltamax (float64), phi(N), hevi(N) before the call;
+! output hevi(N).
 program emx_dropin
   use iso_c_binding
   use dimensiones
@@ -28,7 +32,7 @@ program emx_dropin
   use multiphase
   use consadim
   implicit none
-  character(len=512) :: fin, fout
+  character(len=512) :: fin, fout, mode
   integer(c_int) :: n1, n2, n3
   real(c_double) :: dmin_in, c4_in
   real(8), allocatable :: sgd(:,:,:,:), rs_all(:,:,:,:), rsv_all(:,:,:,:)
@@ -36,6 +40,20 @@ program emx_dropin
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
+  call get_command_argument(3, mode)
+  if (trim(mode) == 'heviside') then
+    open(10, file=trim(fin), access='stream', form='unformatted', status='old')
+    read(10) n1, n2, n3
+    nx = n1; ny = n2; nz = n3
+    allocate(phi(nx,ny,nz), hevi(nx,ny,nz))
+    read(10) deltamax, phi, hevi
+    close(10)
+    call heviside()
+    open(11, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(11) hevi
+    close(11)
+    stop
+  end if
   open(10, file=trim(fin), access='stream', form='unformatted', status='old')
   read(10) n1, n2, n3
   read(10) dmin_in, c4_in

@@ -914,6 +914,21 @@ int orc_ls_reinit_padded(int Nx, int Ny, int Nz, int g, double *phi, const doubl
     return 0;
 }
 
+/* heviside, solver/advectLS.f90:29-59: PI = 2 acos(0) and eps = 1.5
+ * deltamax (:36-37), the three cases in the reference's order (:42-51). */
+int orc_heviside(long n, const double *phi, double deltamax, double band, double *hevi)
+{
+    const double pi = 2.0 * acos(0.0), eps = band * deltamax;
+    if (n < 0) return -1;
+    for (long p = 0; p < n; p++) {
+        const double f = phi[p];
+        if (f < -eps) hevi[p] = 0.0;
+        if (f >= -eps && f <= eps) hevi[p] = 0.5 + f / (2.0 * eps) + sin(pi * f / eps) / (2.0 * pi);
+        if (f > eps) hevi[p] = 1.0;
+    }
+    return 0;
+}
+
 int orc_ls_reinit(int nx, int ny, int nz, double *phi, const double *jbn, const double *dmin,
                   double dx, double dy, double dz, double deltamin)
 {

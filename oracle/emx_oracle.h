@@ -151,6 +151,12 @@ int orc_ls_ghost_mirror(int Nx, int Ny, int Nz, int g, double *a);
 int orc_ls_reinit(int nx, int ny, int nz, double *phi, const double *jbn, const double *dmin,
                   double dx, double dy, double dz, double deltamin);
 
+/* heviside (solver/advectLS.f90:29-59) on n values: eps = band * deltamax
+ * (band 1.5 in the reference; a parameter so tests can plant a mistake),
+ * hevi = 0 below -eps, 1 above eps, 0.5 + phi/(2 eps) + sin(pi phi/eps)/(2 pi)
+ * on the band; NaN phi leaves hevi as it was. */
+int orc_heviside(long n, const double *phi, double deltamax, double band, double *hevi);
+
 /* SSP-RK3 update of sub-step sub (1..3), solver/rungek_ico.f90:87-91,
  * 152-153, 216-219, on n values (um may be NULL for sub-step 1). */
 int orc_rk_update(int sub, long n, double *um1, const double *um0, const double *um,

@@ -66,6 +66,7 @@ def lib():
             "orc_frontera_val": [_I, _I, _I, _P],
             "orc_ls_ghost_mirror": [_I, _I, _I, _I, _P],
             "orc_ls_reinit": [_I, _I, _I, _P, _P, _P, _D, _D, _D, _D],
+            "orc_heviside": [ctypes.c_long, _P, _D, _D, _P],
             "orc_filter_coeffs": [_I, _I, _D, _P, _P, _P],
             "orc_filtronper": [_I, _P, _P, _P, _P, _D],
             "orc_filtrado": [_I, _I, _I, _I, _P, _P, _D, _D, _P, _P, _P],
@@ -374,6 +375,17 @@ def ls_reinit(phi, jbn, dmin, dxyz, deltamin):
     _check(lib().orc_ls_reinit(nx, ny, nz, _ptr(out), _ptr(np.ascontiguousarray(jbn)),
                                _ptr(np.ascontiguousarray(dmin)), dxyz[0], dxyz[1], dxyz[2],
                                deltamin), "ls_reinit")
+    return out
+
+
+def heviside(phi, deltamax, band=1.5, hevi=None):
+    """Smoothed Heaviside; band overrides the 1.5 of eps = 1.5 deltamax to
+    plant a mistake in tests; hevi (default zeros) holds the values a NaN
+    phi leaves unchanged."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.zeros_like(phi) if hevi is None else np.array(hevi, dtype=np.float64, copy=True)
+    _check(lib().orc_heviside(phi.size, _ptr(phi), float(deltamax), float(band), _ptr(out)),
+           "heviside")
     return out
 
 
