@@ -240,9 +240,22 @@ __global__ void k_ls_extrap_list(int stage, const int* list, const int* count, c
   }
 }
 
-// Phi_x_WENO (src/LevelSet/lsTools.cu:9-50).
+// 1/x for x in the normal range: the hardware reciprocal refined by two
+// Newton steps (each doubles the correct bits), within an ulp of the
+// correctly rounded quotient.
+__device__ __forceinline__ double recip_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
+// Phi_x_WENO (src/LevelSet/lsTools.cu:9-50).  The level-set kernels let
+// the compiler fuse multiplies and adds (the library is built without
+// contraction elsewhere): they are issue-bound, and the fused forms change
+// results in the last bits only.
 __device__ __forceinline__ double weno5(double b1, double b2, double b3,
                                         double b4, double b5) {
+#pragma clang fp contract(fast)
   const double s1 = (13.0 / 12.0) * (b1 - 2.0 * b2 + b3) * (b1 - 2.0 * b2 + b3) +
                     (0.25) * (b1 - 4.0 * b2 + 3.0 * b3) * (b1 - 4.0 * b2 + 3.0 * b3);
   const double s2 = (13.0 / 12.0) * (b2 - 2.0 * b3 + b4) * (b2 - 2.0 * b3 + b4) +
@@ -257,7 +270,7 @@ __device__ __forceinline__ double weno5(double b1, double b2, double b3,
   const double d2 = (s2 + 1.0e-6) * (s2 + 1.0e-6);
   const double d3 = (s3 + 1.0e-6) * (s3 + 1.0e-6);
   const double w1 = 0.1 * (d2 * d3), w2 = 0.6 * (d1 * d3), w3 = 0.3 * (d1 * d2);
-  const double isum = 1.0 / (w1 + w2 + w3);
+  const double isum = recip_nr(w1 + w2 + w3);
   const double o1 = w1 * isum;
   const double o2 = w2 * isum;
   const double o3 = w3 * isum;
@@ -295,6 +308,7 @@ __device__ __forceinline__ double bdiff(const double* phi, PDims g, int ax,
 // than 2^32 points, so the wrapped unsigned sum is the exact index.
 __device__ __forceinline__ void axis_bd(const double* src, unsigned stride, int q, int n,
                                         unsigned uid, double ihc, double m, double bd[6]) {
+#pragma clang fp contract(fast)
   double s[7];
 #pragma unroll
   for (int o = 0; o < 7; o++) {
@@ -361,6 +375,7 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
                                                  const double* jbn, PDims g, double iX,
                                                  double iY, double iZ, double dt,
                                                  double* out, double* phi_int, long long id) {
+#pragma clang fp contract(fast)
   const unsigned uid = (unsigned)id, row = uid / (unsigned)g.x;  // NG < 2^32
   const int x = (int)(uid - row * (unsigned)g.x), y = (int)(row % (unsigned)g.y),
             z = (int)(row / (unsigned)g.y);
@@ -399,7 +414,7 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
   double o;
   if (stage == 0) o = sv - dt * rs;
   else if (stage == 1) o = 0.75 * P[id] + 0.25 * (sv - dt * rs);
-  else o = (P[id] + 2.0 * (sv - dt * rs)) / 3.0;
+  else o = (P[id] + 2.0 * (sv - dt * rs)) * (1.0 / 3.0);
   if (phi_int)
     phi_int[(unsigned)(x - GC) + nx * ((unsigned)(y - GC) + ny * (unsigned)(z - GC))] = o;
   else
