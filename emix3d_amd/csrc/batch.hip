@@ -243,7 +243,7 @@ int plan_lines(emx_batch* b, int dir, BatchShape s) {
     blk[i + 1] = blk[i] + L.gx * g.nouter;
     maxthreads = std::max(maxthreads, ((L.T.P * s.lg + 63) / 64) * 64);
     const size_t lds = (size_t)(5 * L.T.n + 1 + 2 * L.T.P * L.T.KB +
-                                ProbGradDiv::NV * (1024 / s.lg) * s.lg) * sizeof(double);
+                                ProbGradDiv::NV * L.T.P * s.lg) * sizeof(double);
     maxlds = std::max(maxlds, lds);
   }
   b->shape[dir] = s;

@@ -382,7 +382,6 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
   constexpr int NV = Prob::NV;
   constexpr int H = scheme_halo<S>::value;
   constexpr int W = C + 2 * H;
-  constexpr int PMAX = 1024 / LG;  // most chunks a workgroup holds
   extern __shared__ double lds[];
   const int n = T.n, P = T.P, K = T.K, KB = T.KB;
   const int ntab = 5 * n + 1;
@@ -393,7 +392,7 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
   double* s_G = s_H + n;
   double* s_wf = s_G + n;       // [P*KB]
   double* s_wb = s_wf + P * KB;  // [P*KB]
-  double* sh = s_wb + P * KB;   // [NV][PMAX][LG]
+  double* sh = s_wb + P * KB;   // [NV][P][LG] chunk carries
   const int tid = threadIdx.x;
   const int li = tid % LG;
   const int c = tid / LG;  // chunk index
@@ -442,8 +441,12 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
       for (int a = 0; a < NV; a++) y[a][q] = 0.0;
     }
   }
+  // work-items past the last chunk (c >= P, in the last wavefront) own no
+  // slot
+  if (c < P) {
 #pragma unroll
-  for (int a = 0; a < NV; a++) sh[(a * PMAX + c) * LG + li] = act ? yp[a] : 0.0;
+    for (int a = 0; a < NV; a++) sh[(a * P + c) * LG + li] = act ? yp[a] : 0.0;
+  }
   __syncthreads();
   if (act) {
     const int kk = c < K ? c : K;
@@ -451,7 +454,7 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
     for (int a = 0; a < NV; a++) {
       double Y = 0.0;
       for (int k = 0; k < kk; k++)
-        Y += s_wf[c * KB + k] * sh[(a * PMAX + c - 1 - k) * LG + li];
+        Y += s_wf[c * KB + k] * sh[(a * P + c - 1 - k) * LG + li];
 #pragma unroll
       for (int q = 0; q < C; q++) {
         const int l = s + q;
@@ -476,8 +479,10 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
       }
     }
   }
+  if (c < P) {
 #pragma unroll
-  for (int a = 0; a < NV; a++) sh[(a * PMAX + c) * LG + li] = act ? xs[a] : 0.0;
+    for (int a = 0; a < NV; a++) sh[(a * P + c) * LG + li] = act ? xs[a] : 0.0;
+  }
   __syncthreads();
   if (!act) return;  // after the last barrier
   const int kb = P - 1 - c < K ? P - 1 - c : K;
@@ -486,7 +491,7 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
   for (int a = 0; a < NV; a++) {
     double acc = 0.0;
     for (int k = 0; k < kb; k++)
-      acc += s_wb[c * KB + k] * sh[(a * PMAX + c + 1 + k) * LG + li];
+      acc += s_wb[c * KB + k] * sh[(a * P + c + 1 + k) * LG + li];
     X[a] = acc;
   }
 #pragma unroll
@@ -914,7 +919,7 @@ hipError_t launch_strided_c(const Prob& prob, const LineTab& T,
     return hipGetLastError();
   } else {
   const size_t lds = (size_t)(5 * T.n + 1 + 2 * T.P * T.KB +
-                              Prob::NV * (1024 / LG) * LG) * sizeof(double);
+                              Prob::NV * T.P * LG) * sizeof(double);
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_lines_strided<S, C, LG, Prob>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
