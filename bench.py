@@ -198,10 +198,11 @@ def main():
 
 # HBM bytes per launch of kernels measured with rocprofv3 --pmc (FETCH_SIZE
 # and WRITE_SIZE, scaled by the same counters' reading for a copy kernel of
-# known size, 1.9998 and 1.0); profiles/rk_stage_256_c6f_hbm_traffic.md
-# holds the runs.
-MEASURED_TRAFFIC = {"flux_project_xi dir0": 6174091167.0, "flux_project": 6173834327.0,
-                    "sgdm": 3741701731.0}
+# known size, 1.9998 and 1.0), keyed by the kernel label (the accumulation
+# passes: the average over their eta and zeta launches);
+# profiles/rk_stage_256_c6f_hbm_traffic.md holds the runs.
+MEASURED_TRAFFIC = {"flux_div_all[accumulate]": 876836941.0, "flux_project_xi": 6174176242.0,
+                    "sgdm": 3727144997.0}
 
 
 def kernel_breakdown(kernels, steps):

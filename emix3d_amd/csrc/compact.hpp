@@ -111,6 +111,15 @@ struct prefetches<Prob, decltype(void(Prob::PREFETCH))> {
   static constexpr bool value = Prob::PREFETCH;
 };
 
+// CHUNK_STORE: the problem scatters a work-item's whole chunk at once
+// (store_w<C>), used where the chunk is contiguous in memory.
+template <class Prob, class = void>
+struct chunk_store { static constexpr bool value = false; };
+template <class Prob>
+struct chunk_store<Prob, decltype(void(Prob::CHUNK_STORE))> {
+  static constexpr bool value = Prob::CHUNK_STORE;
+};
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
@@ -725,10 +734,12 @@ k_lines_tile(Prob prob, LineTab T, int nlanes, long long si, long long so,
 // goes to an LDS tile [n][LG] of the field, and B^T is applied from the
 // tile.  The NV fields of a point are solved one after another.  Launch
 // geometry of k_lines_tile.
-template <int C, int LG, class Prob>
-__global__ void __launch_bounds__(1024)
-k_lines_adj(Prob prob, LineTab T, int nlanes, long long si, long long so,
-            long long sl, double ih) {
+// CHUNK (xi lines, contiguous in memory; problems with CHUNK_STORE): the
+// scatter of a work-item's chunk goes through store_w<C> (k_lines_adj_xi).
+template <int C, int LG, class Prob, bool CHUNK>
+__device__ __forceinline__ void lines_adj_body(const Prob& prob, const LineTab& T, int nlanes,
+                                               long long si, long long so, long long sl,
+                                               double ih) {
   constexpr int NV = Prob::NV;
   constexpr int PMAX = 1024 / LG;
   extern __shared__ double lds[];
@@ -833,11 +844,33 @@ k_lines_adj(Prob prob, LineTab T, int nlanes, long long si, long long so,
     }
   }
   if (!act) return;  // after the last barrier
+  if constexpr (CHUNK) {
+    if (s + C <= n) {  // the chunk's C points are adjacent (unit stride)
+      prob.template store_w<C>(base + s, v);
+      return;
+    }
+  }
 #pragma unroll
   for (int q = 0; q < C; q++) {
     const int l = s + q;
     if (l < n) prob.store(base + (long long)l * sl, v[q]);
   }
+}
+
+template <int C, int LG, class Prob>
+__global__ void __launch_bounds__(1024)
+k_lines_adj(Prob prob, LineTab T, int nlanes, long long si, long long so, long long sl,
+            double ih) {
+  lines_adj_body<C, LG, Prob, false>(prob, T, nlanes, si, so, sl, ih);
+}
+
+// The xi form: at most 256 work-items per workgroup and three waves per
+// SIMD, which leaves the chunk scatter the registers of its C points.
+template <int C, int LG, class Prob>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_lines_adj_xi(Prob prob, LineTab T, int nlanes, long long si, long long so, long long sl,
+               double ih) {
+  lines_adj_body<C, LG, Prob, true>(prob, T, nlanes, si, so, sl, ih);
 }
 
 // ---------------------------------------------------------------------------
@@ -1047,20 +1080,39 @@ int launch_lines_adj(emx_ctx* ctx, int dir, const Prob& prob, double ih) {
   const DirGeom g = dir_geom(ctx->d, dir);
   const LineTab& T = ctx->adj[dir];
   LaunchTimer timer(ctx, Prob::label(), dir, prob.bytes_pt() * (double)ctx->d.N());
-  constexpr int LG = 16;
-  dim3 grid((g.nlanes + LG - 1) / LG, g.nouter);
-  const int threads = ((T.P * LG + 63) / 64) * 64;
-  const size_t lds = (size_t)(5 * T.n + 1 + 2 * T.P * T.KB + 7 * T.n + (1024 / LG) * LG +
-                              T.n * LG) * sizeof(double);
-  auto go = [&](auto kern) {
+  auto go = [&](auto kern, int LG) {
+    dim3 grid((g.nlanes + LG - 1) / LG, g.nouter);
+    const int threads = ((T.P * LG + 63) / 64) * 64;
+    const size_t lds = (size_t)(5 * T.n + 1 + 2 * T.P * T.KB + 7 * T.n + (1024 / LG) * LG +
+                                T.n * LG) * sizeof(double);
     if (lds > 65536)
       (void)hipFuncSetAttribute((const void*)kern,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kern, grid, dim3(threads), lds, ctx->stream, prob, T, g.nlanes,
                        g.si, g.so, g.sl, ih);
   };
-  if (T.C == 4) go(k_lines_adj<4, LG, Prob>);
-  else go(k_lines_adj<8, LG, Prob>);
+  // xi lines of a chunk-scattering problem: as many lines per workgroup
+  // (4, 8 or 16) as keep it within 256 items (at most 64 chunks a line,
+  // adj_chunk), so the chunk scatter has the registers of such a
+  // workgroup; otherwise 16 lines per workgroup
+  if constexpr (chunk_store<Prob>::value) {
+    if (g.sl == 1 && T.P <= 64) {
+      const int lg = T.P <= 16 ? 16 : (T.P <= 32 ? 8 : 4);
+      if (T.C == 4) {
+        if (lg == 16) go(k_lines_adj_xi<4, 16, Prob>, 16);
+        else if (lg == 8) go(k_lines_adj_xi<4, 8, Prob>, 8);
+        else go(k_lines_adj_xi<4, 4, Prob>, 4);
+      } else {
+        if (lg == 16) go(k_lines_adj_xi<8, 16, Prob>, 16);
+        else if (lg == 8) go(k_lines_adj_xi<8, 8, Prob>, 8);
+        else go(k_lines_adj_xi<8, 4, Prob>, 4);
+      }
+      goto launched;
+    }
+  }
+  if (T.C == 4) go(k_lines_adj<4, 16, Prob>, 16);
+  else go(k_lines_adj<8, 16, Prob>, 16);
+launched:
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("transposed line-solver launch (dir %d, C %d) failed: %s", dir, T.C,

@@ -98,6 +98,155 @@ struct ProbDivAllAdj {
 #pragma unroll
     for (int a = 0; a < 3; a++) v[5 + a] = ij * rsv_bar[(1 + a) * N + p];
   }
+  // The scatter of a work-item's W consecutive points p .. p+W-1 of one
+  // line, used where the line is contiguous in memory (the xi pass): in
+  // two phases (convective, viscous), each of which reads all its input
+  // fields for the W points before it computes and writes its outputs, so
+  // every cache line a wavefront touches is used whole by a few consecutive
+  // instructions and the loads of a phase overlap.  (Point by point, each
+  // line of every field was visited once per point, two dozen fields
+  // apart, and was evicted from L2 between the visits.)  Per point the
+  // same arithmetic in the same order as store().
+  static constexpr bool CHUNK_STORE = true;
+  template <int W>
+  __device__ __forceinline__ void putw(double* f, const double* v, bool first) const {
+    if (first) {
+#pragma unroll
+      for (int i = 0; i < W; i++) f[i] = v[i];
+    } else {
+      double o[W];
+#pragma unroll
+      for (int i = 0; i < W; i++) o[i] = f[i];
+#pragma unroll
+      for (int i = 0; i < W; i++) f[i] = o[i] + v[i];
+    }
+  }
+  template <int W>
+  __device__ __forceinline__ void store_w(long long p, const double (*x)[NV]) const {
+    const long long N = in.N;
+    const int d = dir;
+    const bool first = pass == 0;
+    const double* jb = in.jbn + p;
+    double jm[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) jm[i] = jb[9 * N + i] * jb[(4 * d) * N + i];
+    // projected flux adjoint k of point i (convective k < 5, viscous 5..7)
+    auto e = [&](int i, int k) { return jm[i] * x[i][k]; };
+    // --- convective: inputs um(2+d), u(1..3), um5 ---
+    {
+      double umd[W], uu[3][W], um5[W];
+#pragma unroll
+      for (int i = 0; i < W; i++) umd[i] = in.um[(1 + d) * N + p + i];
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int i = 0; i < W; i++) uu[a][i] = in.u[a * N + p + i];
+#pragma unroll
+      for (int i = 0; i < W; i++) um5[i] = in.um[4 * N + p + i];
+      double r[W];
+      // um_bar(2+d) = -e1 - sum_a e(2+a) u(1+a): only this pass writes it
+#pragma unroll
+      for (int i = 0; i < W; i++) {
+        double t = -e(i, 0);
+#pragma unroll
+        for (int a = 0; a < 3; a++) t -= e(i, 1 + a) * uu[a][i];
+        r[i] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < W; i++) um_bar[(1 + d) * N + p + i] = r[i];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+          double g = -e(i, 1 + a) * umd[i];
+          if (a == d) g -= e(i, 4) * um5[i];
+          r[i] = g;
+        }
+        putw<W>(u_bar + a * N + p, r, first);
+      }
+      // u(1+d) by selects (a run-time index would put the arrays in scratch)
+#pragma unroll
+      for (int i = 0; i < W; i++)
+        r[i] = -e(i, 4) * (d == 0 ? uu[0][i] : (d == 1 ? uu[1][i] : uu[2][i]));
+      putw<W>(um_bar + 4 * N + p, r, first);
+#pragma unroll
+      for (int i = 0; i < W; i++) r[i] = -(d == 0 ? e(i, 1) : (d == 1 ? e(i, 2) : e(i, 3)));
+      putw<W>(pres_bar + p, r, first);
+    }
+    // --- viscous: inputs vis, jbn 1/5/9 and the five dvel components of
+    // this direction; pass that first touches dvel component m (1-based):
+    // 1,2,3,4,7 -> 0; 5,6,8 -> 1; 9 -> 2 ---
+    {
+      // the two (metric, gradient) pairs of viscous flux a != d, and the
+      // diagonal one of a == d (metric jbn 1+4a, gradient 1+4a)
+      auto pair = [&](int a, int& jx, int& mx, int& jz, int& mz) {
+        const int lo = a < d ? a : d, hi = a < d ? d : a;
+        if (lo == 0 && hi == 1) { jx = 1; mx = 2; jz = 5; mz = 4; }       // J1*D2 + J5*D4
+        else if (lo == 0) { jx = 1; mx = 3; jz = 9; mz = 7; }             // J1*D3 + J9*D7
+        else { jx = 9; mx = 8; jz = 5; mz = 6; }                          // J9*D8 + J5*D6
+      };
+      double mu[W], J1[W], J5[W], J9[W];
+#pragma unroll
+      for (int i = 0; i < W; i++) mu[i] = in.vis[p + i];
+#pragma unroll
+      for (int i = 0; i < W; i++) J1[i] = jb[i];
+#pragma unroll
+      for (int i = 0; i < W; i++) J5[i] = jb[4 * N + i];
+#pragma unroll
+      for (int i = 0; i < W; i++) J9[i] = jb[8 * N + i];
+      // gradient components: Dp[a][0] the diagonal one of a == d, else
+      // Dp[a][0], Dp[a][1] the pair of flux a (indices fixed at compile
+      // time; d only chooses which loads run)
+      double Dp[3][2][W];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (a == d) {
+#pragma unroll
+          for (int i = 0; i < W; i++) Dp[a][0][i] = in.dvel[(4 * a) * N + p + i];
+        } else {
+          int jx, mx, jz, mz;
+          pair(a, jx, mx, jz, mz);
+#pragma unroll
+          for (int i = 0; i < W; i++) Dp[a][0][i] = in.dvel[(mx - 1) * N + p + i];
+#pragma unroll
+          for (int i = 0; i < W; i++) Dp[a][1][i] = in.dvel[(mz - 1) * N + p + i];
+        }
+      }
+      auto Jv = [&](int jc, int i) { return jc == 1 ? J1[i] : (jc == 5 ? J5[i] : J9[i]); };
+      double vb[W], r[W];
+#pragma unroll
+      for (int i = 0; i < W; i++) vb[i] = 0.0;
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        if (a == d) {
+          const int m = 1 + 4 * a, jc = 1 + 4 * a;
+          const int f0 = (m == 5 || m == 6 || m == 8) ? 1 : (m == 9 ? 2 : 0);
+#pragma unroll
+          for (int i = 0; i < W; i++) {
+            const double w = e(i, 5 + a);
+            vb[i] += w * (2.0 * (Jv(jc, i) * Dp[a][0][i]));
+            r[i] = w * mu[i] * 2.0 * Jv(jc, i);
+          }
+          putw<W>(dvel_bar + (m - 1) * N + p, r, f0 == d);
+        } else {
+          int jx, mx, jz, mz;
+          pair(a, jx, mx, jz, mz);
+#pragma unroll
+          for (int i = 0; i < W; i++)
+            vb[i] += e(i, 5 + a) * (Jv(jx, i) * Dp[a][0][i] + Jv(jz, i) * Dp[a][1][i]);
+          const int fx = (mx == 5 || mx == 6 || mx == 8) ? 1 : (mx == 9 ? 2 : 0);
+          const int fz = (mz == 5 || mz == 6 || mz == 8) ? 1 : (mz == 9 ? 2 : 0);
+#pragma unroll
+          for (int i = 0; i < W; i++) r[i] = e(i, 5 + a) * mu[i] * Jv(jx, i);
+          putw<W>(dvel_bar + (mx - 1) * N + p, r, fx == d);
+#pragma unroll
+          for (int i = 0; i < W; i++) r[i] = e(i, 5 + a) * mu[i] * Jv(jz, i);
+          putw<W>(dvel_bar + (mz - 1) * N + p, r, fz == d);
+        }
+      }
+      putw<W>(vis_bar + p, vb, first);
+    }
+  }
   __device__ __forceinline__ void put(double* f, long long p, double v, bool first) const {
     f[p] = first ? v : f[p] + v;
   }
