@@ -323,6 +323,23 @@ __device__ __forceinline__ void axis_bd(const double* src, unsigned stride, int 
   if (q == 2) bd[0] = bd[1];
 }
 
+// axis_bd for an axis position q with 3 <= q <= n - 4: every stencil
+// position lies inside the grid, so no clamps and no edge rules.
+__device__ __forceinline__ void axis_bd_inner(const double* src, unsigned stride,
+                                              unsigned uid, double ihc, double m,
+                                              double bd[6]) {
+#pragma clang fp contract(fast)
+  // signed offsets from the point itself: on the x axis (stride 1) they
+  // become the loads' immediate offsets
+  const double* c0 = src + uid;
+  const int st = (int)stride;
+  double s[7];
+#pragma unroll
+  for (int o = 0; o < 7; o++) s[o] = c0[(o - 3) * st];
+#pragma unroll
+  for (int o = 0; o < 6; o++) bd[o] = ihc * (s[o + 1] - s[o]) * m;
+}
+
 // The five WENO5 operands of the left-biased side along axis c from bd
 // (PhiDevPlusParameterJB, lsTools.cu:587-673: positions q-2..q+2), with the
 // reference's substitutions at the edges.  On the z axis the last plane's
@@ -404,9 +421,17 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
     const int pos[3] = {x, y, z}, ext[3] = {g.x, g.y, g.z};
     const unsigned stride[3] = {1u, (unsigned)g.x, (unsigned)g.x * (unsigned)g.y};
     double bd[6], b[5];
-    axis_bd(src, stride[c], pos[c], ext[c], uid, ih[c], met[c], bd);
-    if (v > 0.0) stencil_left(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, b);
-    else stencil_right(pos[c], ext[c], bd, b);
+    if (pos[c] >= 3 && pos[c] <= ext[c] - 4) {
+      // away from the padded grid's faces: the interior operands, left
+      // (q-2..q+2) or right (q+3..q-1) biased, with no edge rules
+      axis_bd_inner(src, stride[c], uid, ih[c], met[c], bd);
+#pragma unroll
+      for (int t = 0; t < 5; t++) b[t] = v > 0.0 ? bd[t] : bd[5 - t];
+    } else {
+      axis_bd(src, stride[c], pos[c], ext[c], uid, ih[c], met[c], bd);
+      if (v > 0.0) stencil_left(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, b);
+      else stencil_right(pos[c], ext[c], bd, b);
+    }
     r[c] = v * weno5(b[0], b[1], b[2], b[3], b[4]);
   }
   const double rs = r[0] + r[1] + r[2];
