@@ -111,6 +111,15 @@ struct prefetches<Prob, decltype(void(Prob::PREFETCH))> {
   static constexpr bool value = Prob::PREFETCH;
 };
 
+// BATCH_STORE: the problem stores Q points of a line at once
+// (store_batch<Q>), reading what its stores need for all Q first.
+template <class Prob, class = void>
+struct batch_store { static constexpr bool value = false; };
+template <class Prob>
+struct batch_store<Prob, decltype(void(Prob::BATCH_STORE))> {
+  static constexpr bool value = Prob::BATCH_STORE;
+};
+
 // CHUNK_STORE: the problem scatters a work-item's whole chunk at once
 // (store_w<C>), used where the chunk is contiguous in memory.
 template <class Prob, class = void>
@@ -502,6 +511,36 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
     for (int k = 0; k < kb; k++)
       acc += s_wb[c * KB + k] * sh[(a * P + c + 1 + k) * LG + li];
     X[a] = acc;
+  }
+  if constexpr (batch_store<Prob>::value && C % 4 == 0) {
+    // groups of four points, each stored by one store_batch where the whole
+    // group lies on the line
+#pragma unroll
+    for (int q0 = 0; q0 < C; q0 += 4) {
+      if (s + q0 + 4 <= n) {
+        double v[4][NV];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const double Gl = s_G[s + q0 + i];
+#pragma unroll
+          for (int a = 0; a < NV; a++) v[i][a] = y[a][q0 + i] + Gl * X[a];
+        }
+        prob.template store_batch<4>(base + (long long)(s + q0) * sl, sl, v);
+      } else {
+#pragma unroll
+        for (int q = q0; q < q0 + 4; q++) {
+          const int l = s + q;
+          if (l < n) {
+            const double Gl = s_G[l];
+            double v[NV];
+#pragma unroll
+            for (int a = 0; a < NV; a++) v[a] = y[a][q] + Gl * X[a];
+            prob.store(base + (long long)l * sl, v);
+          }
+        }
+      }
+    }
+    return;
   }
 #pragma unroll
   for (int q = 0; q < C; q++) {

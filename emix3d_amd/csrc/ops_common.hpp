@@ -308,6 +308,40 @@ struct ProbAccum {
     v[0] = f[p];
     v[1] = f[N + p];
   }
+  // Q points p + i*sl at once (a work-item's points along its line): the
+  // accumulator (and 1/J) reads of all Q are issued before the first
+  // write, so their latencies overlap instead of one read-modify-write
+  // waiting for the last.  Same arithmetic as store().
+  static constexpr bool BATCH_STORE = true;
+  template <int Q>
+  __device__ __forceinline__ void store_batch(long long p, long long sl,
+                                              const double (*v)[NV]) const {
+    if (pass == 0) {
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        __builtin_nontemporal_store(v[i][0], &acc[p + i * sl]);
+        __builtin_nontemporal_store(v[i][1], &acc[N + p + i * sl]);
+      }
+      return;
+    }
+    double a0[Q], a1[Q], q[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      a0[i] = acc[p + i * sl];
+      a1[i] = acc[N + p + i * sl];
+      q[i] = pass == 2 ? ij[p + i * sl] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      if (pass == 1) {
+        __builtin_nontemporal_store(a0[i] + v[i][0], &acc[p + i * sl]);
+        __builtin_nontemporal_store(a1[i] + v[i][1], &acc[N + p + i * sl]);
+      } else {
+        __builtin_nontemporal_store(q[i] * (a0[i] + v[i][0]), &out0[p + i * sl]);
+        __builtin_nontemporal_store(q[i] * (a1[i] + v[i][1]), &out1[p + i * sl]);
+      }
+    }
+  }
   // nontemporal stores, as ProbDeriv::store
   __device__ __forceinline__ void store(long long p, const double* v) const {
     if (pass == 0) {
