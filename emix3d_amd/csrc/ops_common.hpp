@@ -68,6 +68,30 @@ struct ProbGradDiv {
     else if (dir == 1) acc[p] = acc[p] + v[3];
     else div[p] = jbn[10 * N + p] * (acc[p] + v[3]);
   }
+  // Q points p + i*sl at once: the accumulator (and 1/J) reads first, as
+  // ProbAccum::store_batch.  Same arithmetic as store().
+  static constexpr bool BATCH_STORE = true;
+  template <int Q>
+  __device__ __forceinline__ void store_batch(long long p, long long sl,
+                                              const double (*v)[NV]) const {
+    double a[Q], q[Q];
+    if (dir > 0) {
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        a[i] = acc[p + i * sl];
+        q[i] = dir == 2 ? jbn[10 * N + p + i * sl] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      const long long pi = p + i * sl;
+#pragma unroll
+      for (int c = 0; c < 3; c++) dvel[(3 * dir + c) * N + pi] = v[i][c];
+      if (dir == 0) acc[pi] = v[i][3];
+      else if (dir == 1) acc[pi] = a[i] + v[i][3];
+      else div[pi] = q[i] * (a[i] + v[i][3]);
+    }
+  }
 };
 
 // Inputs of flujos (solver/flujos_compact.f90) addressed by linear index.
