@@ -24,6 +24,18 @@ struct ProbFilter {
   __device__ __forceinline__ void store(long long p, const double* v) const {
     __builtin_nontemporal_store(v[0] * ijac[p], &f[p]);
   }
+  // Q points p + i*sl at once, the 1/J reads first (each would otherwise
+  // wait for the store before it)
+  static constexpr bool BATCH_STORE = true;
+  template <int Q>
+  __device__ __forceinline__ void store_batch(long long p, long long sl,
+                                              const double (*v)[NV]) const {
+    double q[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) q[i] = ijac[p + i * sl];
+#pragma unroll
+    for (int i = 0; i < Q; i++) __builtin_nontemporal_store(v[i][0] * q[i], &f[p + i * sl]);
+  }
 };
 
 namespace {
