@@ -58,6 +58,16 @@ int op_deriv(emx_ctx* ctx, int dir, const double* f, double* df, double ih) {
 static int deriv_fields(emx_ctx* ctx, int dir, int m, const double* const* in,
                         double* const* out, double ih) {
   int f = 0;
+  if (m == 4 && ctx->plan[PLAN_DERIV][dir].kind == KIND_WAVE) {
+    // the wavefront kind (one line per wave, 256-item workgroups) holds
+    // four fields per line: one launch for the gradient's four
+    ProbDeriv<4> p;
+    for (int a = 0; a < 4; a++) {
+      p.in[a] = in[a];
+      p.out[a] = out[a];
+    }
+    return launch_lines(ctx, dir, p, ih);
+  }
   for (; f + 1 < m; f += 2) {
     ProbDeriv<2> p;
     p.in[0] = in[f];
