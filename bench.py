@@ -201,8 +201,8 @@ def main():
 # known size, 1.9998 and 1.0), keyed by the kernel label (the accumulation
 # passes: the average over their eta and zeta launches);
 # profiles/rk_stage_256_c6f_hbm_traffic.md holds the runs.
-MEASURED_TRAFFIC = {"flux_div_all[accumulate]": 876836941.0, "flux_project_xi": 6174176242.0,
-                    "sgdm": 3727144997.0}
+MEASURED_TRAFFIC = {"flux_div_all[accumulate]": 881288263.0, "flux_project_xi": 6174124426.0,
+                    "sgdm": 3716755246.0}
 
 
 def kernel_breakdown(kernels, steps):
