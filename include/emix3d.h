@@ -19,8 +19,12 @@
  * Two families of entry points:
  *   - emx_* take an emx_ctx and DEVICE pointers; they are asynchronous on
  *     the context's stream, allocate nothing, and can be captured in a HIP
- *     graph.  They return EMX_OK or a negative status; emx_last_error()
- *     describes the last failure of the calling thread.
+ *     graph (tests/test_gpu_graph.py replays the stage and both closures),
+ *     except the set-up calls that report to the host (emx_metrics and
+ *     emx_sgs_geometry wait for their result) and the first emx_filtrado
+ *     per (line length, cfilt), which builds its line tables.  They return
+ *     EMX_OK or a negative status; emx_last_error() describes the last
+ *     failure of the calling thread.
  *   - the reference's own bind(C) symbols (cudaDeriv, cuDivergence,
  *     advect_CUDA) with their exact signatures, taking HOST pointers and
  *     returning after the results are back in host memory, plus host
