@@ -96,6 +96,25 @@ def test_batch_stage_vs_oracle_every_shape(oracle):
             check(oracle, boxes, refs, f"wave C{c} graph={graph}")
 
 
+@pytest.mark.parametrize("scheme,sid", [("c4", 0), ("c6f", 1)])
+def test_batch_stage_other_schemes(oracle, scheme, sid):
+    """The C4 and C6F instantiations of the batched kernels (the cases above
+    run C6C) against the oracle of the same scheme."""
+    b, boxes = make_batch(DIMS, scheme)
+    refs = [reference(oracle, z, sid) for z in boxes]
+    for lg, c in ((16, 4), (32, 8)):
+        for d in range(3):
+            b.set_plan(d, lg, c)
+        for graph in (False, True):
+            reset(boxes)
+            b.stage(graph=graph)
+            check(oracle, boxes, refs, f"{scheme} lg{lg} C{c} graph={graph}")
+    b.set_plan(0, 0, 4)
+    reset(boxes)
+    b.stage(graph=True)
+    check(oracle, boxes, refs, f"{scheme} wave C4")
+
+
 def test_batch_wave_chunk_per_box(oracle):
     # x lines of 20, 70, 130 and 260 points: 1, 2, 4 and 8 points per lane in
     # one launch of the wavefront kind (each box its shortest chunk)
