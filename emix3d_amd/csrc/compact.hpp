@@ -1130,28 +1130,25 @@ int launch_lines_adj(emx_ctx* ctx, int dir, const Prob& prob, double ih) {
     hipLaunchKernelGGL(kern, grid, dim3(threads), lds, ctx->stream, prob, T, g.nlanes,
                        g.si, g.so, g.sl, ih);
   };
-  // xi lines of a chunk-scattering problem: as many lines per workgroup
-  // (4, 8 or 16) as keep it within 256 items (at most 64 chunks a line,
-  // adj_chunk), so the chunk scatter has the registers of such a
-  // workgroup; otherwise 16 lines per workgroup
+  // xi lines of a chunk-scattering problem with chunks of four (lines of up
+  // to 256): as many lines per workgroup (4, 8 or 16) as keep it within
+  // 256 items, so the chunk scatter has the registers of such a workgroup
+  // (with chunks of eight it does not fit them); otherwise 16 lines per
+  // workgroup and the point-by-point scatter
+  bool chunked = false;
   if constexpr (chunk_store<Prob>::value) {
-    if (g.sl == 1 && T.P <= 64) {
+    if (g.sl == 1 && T.C == 4 && T.P <= 64) {
       const int lg = T.P <= 16 ? 16 : (T.P <= 32 ? 8 : 4);
-      if (T.C == 4) {
-        if (lg == 16) go(k_lines_adj_xi<4, 16, Prob>, 16);
-        else if (lg == 8) go(k_lines_adj_xi<4, 8, Prob>, 8);
-        else go(k_lines_adj_xi<4, 4, Prob>, 4);
-      } else {
-        if (lg == 16) go(k_lines_adj_xi<8, 16, Prob>, 16);
-        else if (lg == 8) go(k_lines_adj_xi<8, 8, Prob>, 8);
-        else go(k_lines_adj_xi<8, 4, Prob>, 4);
-      }
-      goto launched;
+      if (lg == 16) go(k_lines_adj_xi<4, 16, Prob>, 16);
+      else if (lg == 8) go(k_lines_adj_xi<4, 8, Prob>, 8);
+      else go(k_lines_adj_xi<4, 4, Prob>, 4);
+      chunked = true;
     }
   }
-  if (T.C == 4) go(k_lines_adj<4, 16, Prob>, 16);
-  else go(k_lines_adj<8, 16, Prob>, 16);
-launched:
+  if (!chunked) {
+    if (T.C == 4) go(k_lines_adj<4, 16, Prob>, 16);
+    else go(k_lines_adj<8, 16, Prob>, 16);
+  }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("transposed line-solver launch (dir %d, C %d) failed: %s", dir, T.C,
