@@ -399,6 +399,15 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
   if (phi_int && (x < GC || x >= g.x - GC || y < GC || y >= g.y - GC || z < GC ||
                   z >= g.z - GC))
     return;  // a ghost point of the last stage: not part of the result
+  // The last stage reads the second stage's output only within 3 points of
+  // the interior (its stencils' reach), so the second stage skips the two
+  // outermost ghost layers (whose values the reference computes and never
+  // uses for the interior).  The first stage computes every padded point:
+  // the second stage's points reach its outermost layer.
+  constexpr int SKIP = GC - 3;
+  if (stage == 1 && (x < SKIP || x >= g.x - SKIP || y < SKIP || y >= g.y - SKIP ||
+                     z < SKIP || z >= g.z - SKIP))
+    return;
   const double ih[3] = {iX, iY, iZ};
   const unsigned nx = (unsigned)(g.x - 2 * GC), ny = (unsigned)(g.y - 2 * GC),
                  nz = (unsigned)(g.z - 2 * GC);
