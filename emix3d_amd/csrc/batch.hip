@@ -32,13 +32,16 @@ struct BatchShape {
 // kind (one wavefront per line, direction 0 only: its lines are contiguous,
 // so each load instruction of a wave reads consecutive points), where each
 // box takes the shortest of the chunk lengths 1, 2, 4, 8 up to C that fits.
-static const BatchShape kBatchShapes[] = {{16, 4}, {16, 8}, {32, 4}, {32, 8}, {64, 8},
-                                          {64, 16}, {0, 1}, {0, 2}, {0, 4}, {0, 8}};
+static const BatchShape kBatchShapes[] = {{8, 4},  {8, 8},   {16, 4}, {16, 8},
+                                          {32, 4}, {32, 8},  {64, 8}, {64, 16},
+                                          {0, 1},  {0, 2},   {0, 4},  {0, 8}};
 constexpr int kWaveLinesPerBlock = 16;  // four per wave
 
 static int lg_kind(int lg) {
   return lg == 0 ? KIND_WAVE
-                 : (lg == 64 ? KIND_STRIDED : (lg == 32 ? KIND_STRIDED32 : KIND_STRIDED16));
+                 : (lg == 64 ? KIND_STRIDED
+                             : (lg == 32 ? KIND_STRIDED32
+                                         : (lg == 16 ? KIND_STRIDED16 : KIND_STRIDED8)));
 }
 
 static size_t wave_lds(int C) {
@@ -63,6 +66,7 @@ static void* batched_wave_kernel(int C) {
 template <int S>
 static void* batched_kernel_s(int lg, int C) {
   if (lg == 0) return batched_wave_kernel<S>(C);
+  if (lg == 8) return C == 4 ? batched_kernel<S, 4, 8>() : batched_kernel<S, 8, 8>();
   if (lg == 16) return C == 4 ? batched_kernel<S, 4, 16>() : batched_kernel<S, 8, 16>();
   if (lg == 32) return C == 4 ? batched_kernel<S, 4, 32>() : batched_kernel<S, 8, 32>();
   return C == 8 ? batched_kernel<S, 8, 64>() : batched_kernel<S, 16, 64>();

@@ -220,9 +220,9 @@ int emx_batch_set_stream(emx_batch* b, void* stream);
 int emx_batch_bind(emx_batch* b, int i, const double* u, const double* jbn, double* phi,
                    double* dvel, double* div, double ihx, double ihy, double ihz,
                    double dx, double dy, double dz, double dt);
-/* Line-solver shape of direction dir: lg lines per wavefront (16, 32, 64)
- * and C points per work-item; built pairs (16,4) (16,8) (32,4) (32,8)
- * (64,8) (64,16).  lg = 0 (direction 0 only) is one wavefront per line,
+/* Line-solver shape of direction dir: lg lines per wavefront (8, 16, 32,
+ * 64) and C points per work-item; built pairs (8,4) (8,8) (16,4) (16,8)
+ * (32,4) (32,8) (64,8) (64,16).  lg = 0 (direction 0 only) is one wavefront per line,
  * each box with the shortest of 1, 2, 4, 8 points per lane up to C that
  * fits its lines (C = 1, 2, 4, 8; every line at most 64*C points): the
  * default for direction 0. */

@@ -24,7 +24,7 @@ from emix3d_amd._lib import EmxError
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
 DIMS = [(6, 7, 5), (24, 9, 13), (17, 20, 8), (9, 9, 30), (33, 12, 11)]
-SHAPES = [(16, 4), (16, 8), (32, 4), (32, 8), (64, 8), (64, 16)]
+SHAPES = [(8, 4), (8, 8), (16, 4), (16, 8), (32, 4), (32, 8), (64, 8), (64, 16)]
 WAVE_SHAPES = [(0, 1), (0, 2), (0, 4), (0, 8)]  # one wavefront per line, direction 0
 
 
