@@ -48,9 +48,11 @@ __device__ __forceinline__ void point_features(const double* dvel, const double*
 // Packs the parameters into per-lane BF16 A operands:
 //   A1[mt][lane][8]      layer 1 (features padded to K = 32)
 //   A2[mt][s][lane][8], A3[mt][s][lane][8]   layers 2, 3
+//   A4[s][lane][8]       the output neuron's weights (divided by the scale
+//                        of the hidden layers) in every row
 // lane l holds row m = l%16 (output neuron 16mt+m), K = 8(l/16) + j.
 __global__ void k_mlp_pack(const float* P, const float* norm, __bf16* A1, __bf16* A2,
-                           __bf16* A3) {
+                           __bf16* A3, __bf16* A4) {
   const int l = threadIdx.x;  // 64 lanes
   const int m = l % 16, g = l / 16;
   for (int mt = 0; mt < 4; mt++) {
@@ -64,59 +66,66 @@ __global__ void k_mlp_pack(const float* P, const float* norm, __bf16* A1, __bf16
         const int in = perm_in(s, g, j);
         A2[((mt * 2 + s) * 64 + l) * 8 + j] = (__bf16)P[MW2 + o * MLP_H + in];
         A3[((mt * 2 + s) * 64 + l) * 8 + j] = (__bf16)P[MW3 + o * MLP_H + in];
+        if (mt == 0) A4[(s * 64 + l) * 8 + j] = (__bf16)(P[MW4 + in] / MLP_YS);
       }
     }
   }
 }
 
 // nu_t of every grid point: persistent wavefronts, each taking groups of
-// 64 consecutive points (one per lane, loads coalesced) and keeping the
-// next group's 13 inputs in flight while the network runs on the current.
-__global__ void __launch_bounds__(256)
+// 64 consecutive points (one per lane, loads coalesced).  Three wavefronts
+// per SIMD (at most 168 registers, no spills) hide one group's loads behind
+// the others' networks; keeping the next group's inputs in registers
+// instead allows only two and was measured slower (0.97 against 0.88 ms at
+// 256^3).  Grids hold at most 2^28 points (emx_create), so every access is
+// a uniform field base plus a 32-bit byte offset: no 64-bit address
+// arithmetic per load.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_mlp_infer(const double* dvel, const double* jbn, const double* delta, long long N,
             const float* P, const float* norm, const __bf16* A, double* nut, double c4,
             double* vis) {
   __shared__ __attribute__((aligned(16))) float vec[MLP_VEC + 2];
+  __shared__ double nrm[2 * MLP_IN];  // in_shift, in_scale
   __shared__ float xs[4][MLP_IN * 64];
   __shared__ bf16x8 lw[MLP_LW];
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   mlp_stage_weights(lw, A);
   mlp_stage_vectors(vec, P, norm);
+  for (int t = threadIdx.x; t < 2 * MLP_IN; t += blockDim.x) nrm[t] = (double)norm[t];
   __syncthreads();
   const long long ngroup = (N + 63) / 64;
   const long long nwave = (long long)gridDim.x * (blockDim.x >> 6);
-  long long grp = (long long)blockIdx.x * (blockDim.x >> 6) + w;
-  // raw inputs of one point: dvel 1..9, jbn 1, 5, 9, delta
-  auto load = [&](long long gr, double r[13]) {
-    long long p = gr * 64 + l;
-    p = p < N ? p : N - 1;
+  for (long long grp = (long long)blockIdx.x * (blockDim.x >> 6) + w; grp < ngroup;
+       grp += nwave) {
+    const long long p = grp * 64 + l;
+    const unsigned off = (unsigned)(p < N ? p : N - 1) * 8u;
+    auto at = [off](const double* base) -> const double& {
+      return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
+    };
+    // raw inputs of the point: dvel 1..9, jbn 1, 5, 9, delta
+    double r[13];
 #pragma unroll
-    for (int c = 0; c < 9; c++) r[c] = dvel[c * N + p];
-    r[9] = jbn[p];
-    r[10] = jbn[4 * N + p];
-    r[11] = jbn[8 * N + p];
-    r[12] = delta[p];
-  };
-  double nxt[13];
-  if (grp < ngroup) load(grp, nxt);
-  for (; grp < ngroup; grp += nwave) {
-    double cur[13];
-#pragma unroll
-    for (int c = 0; c < 13; c++) cur[c] = nxt[c];
-    if (grp + nwave < ngroup) load(grp + nwave, nxt);
+    for (int c = 0; c < 9; c++) r[c] = at(dvel + c * N);
+    r[9] = at(jbn);
+    r[10] = at(jbn + 4 * N);
+    r[11] = at(jbn + 8 * N);
+    r[12] = at(delta);
     float x[MLP_IN];
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
       for (int b = 0; b < 3; b++)
-        x[3 * a + b] = (float)((cur[9 + b] * cur[3 * b + a] - norm[3 * a + b]) *
-                               norm[MLP_IN + 3 * a + b]);
-    x[9] = (float)((cur[12] - norm[9]) * norm[MLP_IN + 9]);
+        x[3 * a + b] =
+            (float)((r[9 + b] * r[3 * b + a] - nrm[3 * a + b]) * nrm[MLP_IN + 3 * a + b]);
+    x[9] = (float)((r[12] - nrm[9]) * nrm[MLP_IN + 9]);
     const float nu = mlp_group64(lw, x, xs[w], vec, l);
-    const long long p = grp * 64 + l;
     if (p < N) {
-      nut[p] = (double)nu;
-      if (vis) vis[p] = c4 * 1.0 + (double)nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
+      double* const np = reinterpret_cast<double*>(reinterpret_cast<char*>(nut) + off);
+      *np = (double)nu;
+      if (vis) {
+        double* const vp = reinterpret_cast<double*>(reinterpret_cast<char*>(vis) + off);
+        *vp = c4 * 1.0 + (double)nu;  // viscosidad (solver/viscosidad_incomp.f90:15-17)
+      }
     }
   }
 }
@@ -302,13 +311,14 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, int n, floa
 
 // ---------------------------------------------------------------------------
 
-size_t mlp_pack_elems() { return (size_t)(4 + 8 + 8) * 64 * 8; }
+size_t mlp_pack_elems() { return (size_t)MLP_PACK_ELEMS; }
 
 int op_mlp_pack(emx_ctx* ctx, const float* params, const float* norm) {
   __bf16* A = reinterpret_cast<__bf16*>(ctx->mlp_pack);
   EMX_TIMED(ctx, "mlp_pack", 0.0,
             hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, ctx->stream, params, norm,
-                               A + MLP_PACK_A1, A + MLP_PACK_A2, A + MLP_PACK_A3));
+                               A + MLP_PACK_A1, A + MLP_PACK_A2, A + MLP_PACK_A3,
+                               A + MLP_PACK_A4));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }

@@ -12,8 +12,9 @@
 // already that lane's B operand of the next layer once the K order of the
 // next layer's weights is permuted to match (perm_in): hidden activations
 // never leave the lane.  The features go in through a wave-private LDS
-// transpose (10 x 64 floats), and the output reduction leaves lane l with
-// the value of point l of the group.
+// transpose (10 x 64 floats), and the output neuron, also a matrix-core
+// product (its weights in every row), leaves lane l with the value of
+// point l of the group.
 #pragma once
 #include "compact.hpp"
 
@@ -29,10 +30,11 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 
 // Packed A operands (k_mlp_pack): A1[mt][lane][8] (layer 1, features
-// padded to K = 32), A2[mt][s][lane][8], A3[mt][s][lane][8]; lane l holds
-// row m = l%16 (neuron 16mt+m), K = 8(l/16) + j.
+// padded to K = 32), A2[mt][s][lane][8], A3[mt][s][lane][8], A4[s][lane][8]
+// (the output neuron, its weights in all 16 rows); lane l holds row
+// m = l%16 (neuron 16mt+m), K = 8(l/16) + j.
 constexpr int MLP_PACK_A1 = 0, MLP_PACK_A2 = 4 * 64 * 8, MLP_PACK_A3 = MLP_PACK_A2 + 8 * 64 * 8,
-              MLP_PACK_ELEMS = MLP_PACK_A3 + 8 * 64 * 8;
+              MLP_PACK_A4 = MLP_PACK_A3 + 8 * 64 * 8, MLP_PACK_ELEMS = MLP_PACK_A4 + 2 * 64 * 8;
 
 // Input neuron of K index k = 8g + j (lane group g, element j) in K-step s
 // of layers 2 and 3: the lane's own result elements of output tiles 2s and
@@ -52,14 +54,26 @@ constexpr float MLP_YS = -1.44269504088896340736f;
 __device__ __forceinline__ float silu_scaled(float y) {
   return y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y));
 }
+// softplus(z) = max(z, 0) + log1p(e), e = exp(-|z|) in (0, 1], with the
+// hardware base-2 exponential and logarithm: log1p(e) = ln(u) * e / (u - 1)
+// for u = 1 + e rounded (Goldberg's correction: a few ulp), and e itself
+// where u rounds to 1.  One point per lane, 14 instructions instead of the
+// ~100 of the library's log1pf.
 __device__ __forceinline__ float softplus_fast(float z) {
-  return z > 20.0f ? z : log1pf(__expf(z));
+  const float e = __builtin_amdgcn_exp2f(-1.44269504088896340736f * __builtin_fabsf(z));
+  const float u = 1.0f + e;
+  const float d = u - 1.0f;
+  const float lp = d == 0.0f ? e
+                             : 0.69314718055994530942f * __builtin_amdgcn_logf(u) *
+                                   (e * __builtin_amdgcn_rcpf(d));
+  return __builtin_fmaxf(z, 0.0f) + lp;
 }
 
-// The packed A operands staged in LDS by a workgroup (20 KB, MLP_PACK_ELEMS
+// The packed A operands staged in LDS by a workgroup (22 KB, MLP_PACK_ELEMS
 // BF16 values in units of bf16x8: layer 1 at 0, layer 2 at 256, layer 3 at
-// 768).  A wavefront reads each operand once per layer for all four point
-// tiles, so the weights take no registers between layers.
+// 768, the output neuron at 1280).  A wavefront reads a layer's operands
+// from LDS for each point tile, so weights hold few registers (three
+// wavefronts per SIMD fit).
 constexpr int MLP_LW = MLP_PACK_ELEMS / 8;
 __device__ __forceinline__ void mlp_stage_weights(bf16x8* lw, const __bf16* A) {
   const bf16x8* src = reinterpret_cast<const bf16x8*>(A);
@@ -67,8 +81,8 @@ __device__ __forceinline__ void mlp_stage_weights(bf16x8* lw, const __bf16* A) {
 }
 
 // LDS copy of the vectors of the network, shared by a workgroup, in the
-// scaled form: s b1[64], s b2[64], s b3[64], w4[64] / s, then b4 and
-// out_scale.
+// scaled form: s b1[64], s b2[64], s b3[64], w4[64] / s (unused by the
+// matrix-core output layer), then b4 and out_scale.
 constexpr int MLP_VEC = 4 * MLP_H + 2;
 __device__ __forceinline__ void mlp_stage_vectors(float* vec, const float* P, const float* norm) {
   for (int t = threadIdx.x; t < MLP_VEC; t += blockDim.x) {
@@ -98,14 +112,16 @@ __device__ __forceinline__ float mlp_group64(const bf16x8* lw, const float f[MLP
 #pragma unroll
   for (int k = 0; k < MLP_IN; k++) xs[k * 64 + l] = f[k];
   wave_sync();
-  float h[4][4][4];  // [point tile t][neuron tile mt][i]
-  bf16x8 a1[4];
-#pragma unroll
-  for (int mt = 0; mt < 4; mt++) a1[mt] = lw[mt * 64 + l];
+  // activations, already the next layer's B operands: hb[t][s] holds
+  // neurons 16(2s) + 4g + 0..3 and 16(2s+1) + 4g + 0..3 of point 16t + n
+  bf16x8 hb[4][2];
   // layer 1: B operand of tile t = features 8g..8g+7 of point 16t+n (K
   // padded with zeros to 32)
 #pragma unroll
   for (int t = 0; t < 4; t++) {
+    bf16x8 a1[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) a1[mt] = lw[mt * 64 + l];
     bf16x8 x;
 #pragma unroll
     for (int j = 0; j < 8; j++) {
@@ -119,26 +135,21 @@ __device__ __forceinline__ float mlp_group64(const bf16x8* lw, const float f[MLP
       f32x4 acc = mlp_vec4(vec, 0, mt, g);
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[mt], x, acc, 0, 0, 0);
 #pragma unroll
-      for (int i = 0; i < 4; i++) h[t][mt][i] = silu_scaled(acc[i]);
+      for (int i = 0; i < 4; i++) hb[t][mt >> 1][4 * (mt & 1) + i] = (__bf16)silu_scaled(acc[i]);
     }
   }
   wave_sync();  // xs is rewritten by the next group
   // layers 2 and 3
 #pragma unroll
   for (int layer = 0; layer < 2; layer++) {
-    bf16x8 a[4][2];
-#pragma unroll
-    for (int mt = 0; mt < 4; mt++)
-#pragma unroll
-      for (int s = 0; s < 2; s++) a[mt][s] = lw[256 + layer * 512 + (mt * 2 + s) * 64 + l];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      bf16x8 bop[2];
+      bf16x8 a[4][2];
 #pragma unroll
-      for (int s = 0; s < 2; s++)
+      for (int mt = 0; mt < 4; mt++)
 #pragma unroll
-        for (int j = 0; j < 8; j++)
-          bop[s][j] = (__bf16)(j < 4 ? h[t][2 * s][j] : h[t][2 * s + 1][j - 4]);
+        for (int s = 0; s < 2; s++) a[mt][s] = lw[256 + layer * 512 + (mt * 2 + s) * 64 + l];
+      const bf16x8 bop[2] = {hb[t][0], hb[t][1]};
 #pragma unroll
       for (int mt = 0; mt < 4; mt++) {
         f32x4 acc = mlp_vec4(vec, 1 + layer, mt, g);
@@ -146,27 +157,24 @@ __device__ __forceinline__ float mlp_group64(const bf16x8* lw, const float f[MLP
         for (int s = 0; s < 2; s++)
           acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt][s], bop[s], acc, 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; i++) h[t][mt][i] = silu_scaled(acc[i]);
+        for (int i = 0; i < 4; i++)
+          hb[t][mt >> 1][4 * (mt & 1) + i] = (__bf16)silu_scaled(acc[i]);
       }
     }
   }
-  // output neuron: FP32 dot product over the 64 neurons of each point, 16
-  // per lane, summed over the four lane groups; lane l keeps point l
-  float z[4];
+  // output neuron on the matrix cores: with w4 / s in every row of A, every
+  // row of the 16x16 result holds z of the tile's 16 points, so the lane of
+  // point 16t + n holds it for t = its own lane group g: lane l keeps point l
+  float zs = 0.0f;
+  const bf16x8 a4[2] = {lw[1280 + l], lw[1280 + 64 + l]};
 #pragma unroll
   for (int t = 0; t < 4; t++) {
-    float s = 0.0f;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int mt = 0; mt < 4; mt++) {
-      const f32x4 w = mlp_vec4(vec, 3, mt, g);
-#pragma unroll
-      for (int i = 0; i < 4; i++) s = __builtin_fmaf(w[i], h[t][mt][i], s);
-    }
-    s += __shfl_xor(s, 16);
-    s += __shfl_xor(s, 32);
-    z[t] = s;
+    for (int s = 0; s < 2; s++)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a4[s], hb[t][s], acc, 0, 0, 0);
+    zs = g == t ? acc[0] : zs;
   }
-  const float zs = g == 0 ? z[0] : (g == 1 ? z[1] : (g == 2 ? z[2] : z[3]));
   return vec[4 * MLP_H + 1] * softplus_fast(zs + vec[4 * MLP_H]);
 }
 
