@@ -28,8 +28,7 @@ HIPCC_FLAGS = [
 
 
 # Per-source additions.  closure.hip: the matrix cores' results in ordinary
-# VGPRs, so the inference kernel copies nothing out of AGPRs (not used for
-# ops_flux.hip, whose fused closure variant holds its spills in AGPRs).
+# VGPRs, so the inference kernel copies nothing out of AGPRs.
 SOURCE_FLAGS = {"closure.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
