@@ -45,31 +45,31 @@ __device__ __forceinline__ void point_features(const double* dvel, const double*
   f[9] = delta[p];
 }
 
-// Packs the parameters into per-lane BF16 A operands:
+// Packs the parameters into per-lane BF16 A operands, one work-item per
+// packed value (MLP_PACK_ELEMS of them, in units of 64 lanes x 8):
 //   A1[mt][lane][8]      layer 1 (features padded to K = 32)
 //   A2[mt][s][lane][8], A3[mt][s][lane][8]   layers 2, 3
 //   A4[s][lane][8]       the output neuron's weights (divided by the scale
 //                        of the hidden layers) in every row
 // lane l holds row m = l%16 (output neuron 16mt+m), K = 8(l/16) + j.
-__global__ void k_mlp_pack(const float* P, const float* norm, __bf16* A1, __bf16* A2,
-                           __bf16* A3, __bf16* A4) {
-  const int l = threadIdx.x;  // 64 lanes
+__global__ void k_mlp_pack(const float* P, __bf16* A) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= MLP_PACK_ELEMS) return;
+  const int j = e & 7, l = (e >> 3) & 63, unit = e >> 9;
   const int m = l % 16, g = l / 16;
-  for (int mt = 0; mt < 4; mt++) {
-    const int o = 16 * mt + m;
-    for (int j = 0; j < 8; j++) {
-      const int k = 8 * g + j;
-      // layer 1 in the scaled form of mlp_group64: y = MLP_YS * z
-      A1[(mt * 64 + l) * 8 + j] =
-          (__bf16)(k < MLP_IN ? MLP_YS * P[MW1 + o * MLP_IN + k] : 0.0f);
-      for (int s = 0; s < 2; s++) {
-        const int in = perm_in(s, g, j);
-        A2[((mt * 2 + s) * 64 + l) * 8 + j] = (__bf16)P[MW2 + o * MLP_H + in];
-        A3[((mt * 2 + s) * 64 + l) * 8 + j] = (__bf16)P[MW3 + o * MLP_H + in];
-        if (mt == 0) A4[(s * 64 + l) * 8 + j] = (__bf16)(P[MW4 + in] / MLP_YS);
-      }
-    }
+  float v;
+  if (e < MLP_PACK_A2) {
+    // layer 1 in the scaled form of mlp_group64: y = MLP_YS * z
+    const int k = 8 * g + j;
+    v = k < MLP_IN ? MLP_YS * P[MW1 + (16 * unit + m) * MLP_IN + k] : 0.0f;
+  } else if (e < MLP_PACK_A4) {
+    const int layer = e < MLP_PACK_A3 ? 0 : 1;
+    const int r = unit - 4 - 8 * layer, mt = r >> 1, sk = r & 1;
+    v = P[(layer == 0 ? MW2 : MW3) + (16 * mt + m) * MLP_H + perm_in(sk, g, j)];
+  } else {
+    v = P[MW4 + perm_in(unit - 20, g, j)] / MLP_YS;
   }
+  A[e] = (__bf16)v;
 }
 
 // nu_t of every grid point: persistent wavefronts, each taking groups of
@@ -316,9 +316,8 @@ size_t mlp_pack_elems() { return (size_t)MLP_PACK_ELEMS; }
 int op_mlp_pack(emx_ctx* ctx, const float* params, const float* norm) {
   __bf16* A = reinterpret_cast<__bf16*>(ctx->mlp_pack);
   EMX_TIMED(ctx, "mlp_pack", 0.0,
-            hipLaunchKernelGGL(k_mlp_pack, dim3(1), dim3(64), 0, ctx->stream, params, norm,
-                               A + MLP_PACK_A1, A + MLP_PACK_A2, A + MLP_PACK_A3,
-                               A + MLP_PACK_A4));
+            hipLaunchKernelGGL(k_mlp_pack, dim3((MLP_PACK_ELEMS + 255) / 256), dim3(256), 0,
+                               ctx->stream, params, A));
   EMX_TRY_HIP(hipGetLastError());
   return EMX_OK;
 }
