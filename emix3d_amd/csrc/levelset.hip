@@ -513,12 +513,21 @@ __global__ void k_ls_reinit_stage(int stage, const double* src, const double* P,
     for (int c = 0; c < 3; c++) {
       const int pos[3] = {x, y, z}, ext[3] = {g.x, g.y, g.z};
       const unsigned stride[3] = {1u, (unsigned)g.x, (unsigned)g.x * (unsigned)g.y};
-      double bd[6], b[5];
-      axis_bd(src, stride[c], pos[c], ext[c], uid, ih[c], met[c], bd);
-      stencil_left(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, b);
-      const double L = weno5(b[0], b[1], b[2], b[3], b[4]);
-      stencil_right(pos[c], ext[c], bd, b);
-      const double R = weno5(b[0], b[1], b[2], b[3], b[4]);
+      double bd[6], L, R;
+      if (pos[c] >= 3 && pos[c] <= ext[c] - 4) {
+        // away from the padded grid's faces: the interior operands, left
+        // (q-2..q+2) and right (q+3..q-1) biased, with no edge rules
+        axis_bd_inner(src, stride[c], uid, ih[c], met[c], bd);
+        L = weno5(bd[0], bd[1], bd[2], bd[3], bd[4]);
+        R = weno5(bd[5], bd[4], bd[3], bd[2], bd[1]);
+      } else {
+        double b[5];
+        axis_bd(src, stride[c], pos[c], ext[c], uid, ih[c], met[c], bd);
+        stencil_left(src, g, c, pos[c], ext[c], x, y, z, ih[c], met[c], bd, b);
+        L = weno5(b[0], b[1], b[2], b[3], b[4]);
+        stencil_right(pos[c], ext[c], bd, b);
+        R = weno5(b[0], b[1], b[2], b[3], b[4]);
+      }
       const double lp = L > 0.0 ? L : 0.0, rm = R < 0.0 ? R : 0.0;
       double hi = lp * lp, lo = rm * rm;
       r[c] = 0.5 * (ta + 1.0) * (hi > lo ? hi : lo);
