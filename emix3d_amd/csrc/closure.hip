@@ -325,6 +325,10 @@ int op_mlp_pack(emx_ctx* ctx, const float* params, const float* norm) {
 int op_mlp_nut(emx_ctx* ctx, const float* params, const float* norm, const double* dvel,
                const double* jbn, const double* delta, double* nut, double c4, double* vis) {
   const long long N = ctx->d.N();
+  if (N > (1LL << 29)) {  // k_mlp_infer's 32-bit byte offsets
+    set_error("emx_mlp_nut: %lld points, at most 2^29", N);
+    return EMX_ERR_SIZE;
+  }
   __bf16* A = reinterpret_cast<__bf16*>(ctx->mlp_pack);
   EMX_TRY(op_mlp_pack(ctx, params, norm));
   static const int resident = resident_blocks(k_mlp_infer, 0);
