@@ -48,9 +48,11 @@ static size_t wave_lds(int C) {
   return (size_t)(5 * 64 * C + 768 + 4 * 64 * (C + 1)) * sizeof(double);
 }
 
+// small: every workgroup of the pass has at most 256 work-items.
 template <int S, int C, int LG>
-static void* batched_kernel() {
-  return (void*)k_lines_batched<S, C, LG, ProbGradDiv>;
+static void* batched_kernel(bool small) {
+  return small ? (void*)k_lines_batched<S, C, LG, ProbGradDiv, 256>
+               : (void*)k_lines_batched<S, C, LG, ProbGradDiv, 1024>;
 }
 
 template <int S>
@@ -64,19 +66,22 @@ static void* batched_wave_kernel(int C) {
 }
 
 template <int S>
-static void* batched_kernel_s(int lg, int C) {
+static void* batched_kernel_s(int lg, int C, bool small) {
   if (lg == 0) return batched_wave_kernel<S>(C);
-  if (lg == 8) return C == 4 ? batched_kernel<S, 4, 8>() : batched_kernel<S, 8, 8>();
-  if (lg == 16) return C == 4 ? batched_kernel<S, 4, 16>() : batched_kernel<S, 8, 16>();
-  if (lg == 32) return C == 4 ? batched_kernel<S, 4, 32>() : batched_kernel<S, 8, 32>();
-  return C == 8 ? batched_kernel<S, 8, 64>() : batched_kernel<S, 16, 64>();
+  if (lg == 8) return C == 4 ? batched_kernel<S, 4, 8>(small) : batched_kernel<S, 8, 8>(small);
+  if (lg == 16) return C == 4 ? batched_kernel<S, 4, 16>(small) : batched_kernel<S, 8, 16>(small);
+  if (lg == 32) return C == 4 ? batched_kernel<S, 4, 32>(small) : batched_kernel<S, 8, 32>(small);
+  return C == 8 ? batched_kernel<S, 8, 64>(small) : batched_kernel<S, 16, 64>(small);
 }
 
-static void* batched_kernel_for(int scheme, int lg, int C) {
+// threads: the pass's workgroup size (the strided kinds' bound: at most
+// 256 selects the kernel compiled for 256 work-items).
+static void* batched_kernel_for(int scheme, int lg, int C, int threads) {
+  const bool small = threads <= 256;
   switch (scheme) {
-    case EMX_SCHEME_C4: return batched_kernel_s<EMX_SCHEME_C4>(lg, C);
-    case EMX_SCHEME_C6F: return batched_kernel_s<EMX_SCHEME_C6F>(lg, C);
-    default: return batched_kernel_s<EMX_SCHEME_C6C>(lg, C);
+    case EMX_SCHEME_C4: return batched_kernel_s<EMX_SCHEME_C4>(lg, C, small);
+    case EMX_SCHEME_C6F: return batched_kernel_s<EMX_SCHEME_C6F>(lg, C, small);
+    default: return batched_kernel_s<EMX_SCHEME_C6C>(lg, C, small);
   }
 }
 
@@ -207,7 +212,7 @@ int plan_wave_lines(emx_batch* b, BatchShape s) {
   b->nblk[0] = blk[b->nbox];
   b->threads[0] = 256;
   b->lds[0] = wave_lds(s.C);
-  void* k = batched_kernel_for(b->scheme, 0, s.C);
+  void* k = batched_kernel_for(b->scheme, 0, s.C, 256);
   if (b->lds[0] > 65536)
     EMX_TRY_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)b->lds[0]));
@@ -254,7 +259,7 @@ int plan_lines(emx_batch* b, int dir, BatchShape s) {
   b->nblk[dir] = blk[b->nbox];
   b->threads[dir] = maxthreads;
   b->lds[dir] = maxlds;
-  void* k = batched_kernel_for(b->scheme, s.lg, s.C);
+  void* k = batched_kernel_for(b->scheme, s.lg, s.C, maxthreads);
   if (maxlds > 65536)
     EMX_TRY_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)maxlds));
   EMX_TRY_HIP(hipMemcpy(b->lines_d[dir], H.data(), H.size() * sizeof(H[0]),
@@ -266,7 +271,8 @@ int plan_lines(emx_batch* b, int dir, BatchShape s) {
 }
 
 int launch_pass(emx_batch* b, int dir) {
-  void* k = batched_kernel_for(b->scheme, b->shape[dir].lg, b->shape[dir].C);
+  void* k = batched_kernel_for(b->scheme, b->shape[dir].lg, b->shape[dir].C,
+                               b->threads[dir]);
   const void* grids = b->shape[dir].lg == 0 ? (const void*)b->waves_d : (const void*)b->lines_d[dir];
   const int* blk = b->blk_d[dir];
   int ngrid = b->nbox;

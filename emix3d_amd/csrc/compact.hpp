@@ -596,8 +596,11 @@ __device__ __forceinline__ int xcd_run_group(int local, int ngroup) {
   return base + (r & 7) * k + (r >> 3);
 }
 
-template <int S, int C, int LG, class Prob>
-__global__ void __launch_bounds__(1024)
+// MAXT: the largest workgroup the launch uses.  1024 serves lines of up to
+// 512 points at any shape; 256 (lines of up to 256/LG chunks) lets a
+// shape keep its registers instead of spilling to fit 1024 items.
+template <int S, int C, int LG, class Prob, int MAXT = 1024>
+__global__ void __launch_bounds__(MAXT)
 k_lines_batched(const BatchLines<Prob>* grids, const int* blk_start, int ngrid) {
   const int b = blockIdx.x;
   int lo = 0, hi = ngrid;  // blk_start[lo] <= b < blk_start[hi]
