@@ -184,9 +184,9 @@ __device__ __forceinline__ void stage_wave_tabs(const LineTab& T, double* lds) {
     double val;
     if (t < 5 * TS) {
       const int tb = t / TS, r = t % TS, l = (r % 64) * C + r / 64;
-      val = l < n ? T.blob[tb * n + (tb >= 2 ? 1 : 0) + l] : 0.0;
+      val = l < n ? gref(T.blob + tb * n + (tb >= 2 ? 1 : 0) + l) : 0.0;
     } else {
-      val = T.Af[t - 5 * TS];
+      val = gref(T.Af + (t - 5 * TS));
     }
     lds[t] = val;
   }
@@ -435,7 +435,7 @@ __device__ __forceinline__ void lines_strided_body(const Prob& prob, const LineT
     const int nstage = ntab + 2 * P * KB;
     const int woff = (int)(T.WfB - T.blob);
     for (int t = tid; t < nstage; t += blockDim.x)
-      lds[t] = T.blob[t < ntab ? t : t - ntab + woff];
+      lds[t] = gref(T.blob + (t < ntab ? t : t - ntab + woff));
   }
   __syncthreads();
 
@@ -640,18 +640,19 @@ k_lines_batched_wave(const BatchWaveLines<Prob>* grids, const int* blk_start, in
     else hi = mid;
   }
   const BatchWaveLines<Prob>& G = grids[lo];
+  const Prob& prob = G.prob;
   const int blk = b - blk_start[lo];
   if (CMAX >= 8 && G.T.C == 8)
-    lines_wave_body<S, (CMAX >= 8 ? 8 : 1)>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
+    lines_wave_body<S, (CMAX >= 8 ? 8 : 1)>(prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
                                              G.ih, blk, G.nblk);
   else if (CMAX >= 4 && G.T.C == 4)
-    lines_wave_body<S, (CMAX >= 4 ? 4 : 1)>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
+    lines_wave_body<S, (CMAX >= 4 ? 4 : 1)>(prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
                                              G.ih, blk, G.nblk);
   else if (CMAX >= 2 && G.T.C == 2)
-    lines_wave_body<S, (CMAX >= 2 ? 2 : 1)>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
+    lines_wave_body<S, (CMAX >= 2 ? 2 : 1)>(prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
                                              G.ih, blk, G.nblk);
   else
-    lines_wave_body<S, 1>(G.prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl, G.ih, blk, G.nblk);
+    lines_wave_body<S, 1>(prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl, G.ih, blk, G.nblk);
 }
 
 // ---------------------------------------------------------------------------
@@ -698,7 +699,7 @@ k_lines_tile(Prob prob, LineTab T, int nlanes, long long si, long long so,
     const int nstage = ntab + 2 * P * KB;
     const int woff = (int)(T.WfB - T.blob);
     for (int t = tid; t < nstage; t += blockDim.x)
-      lds[t] = T.blob[t < ntab ? t : t - ntab + woff];
+      lds[t] = gref(T.blob + (t < ntab ? t : t - ntab + woff));
   }
 #pragma unroll
   for (int a = 0; a < NV; a++) {
@@ -816,7 +817,7 @@ __device__ __forceinline__ void lines_adj_body(const Prob& prob, const LineTab& 
     const int nstage = ntab + 2 * P * KB;
     const int woff = (int)(T.WfB - T.blob);
     for (int t = tid; t < nstage; t += blockDim.x)
-      lds[t] = T.blob[t < ntab ? t : t - ntab + woff];
+      lds[t] = gref(T.blob + (t < ntab ? t : t - ntab + woff));
     for (int t = tid; t < 7 * n; t += blockDim.x) s_bt[t] = T.BT[t];
   }
 #pragma unroll

@@ -19,6 +19,17 @@
 
 namespace emx {
 
+// A pointer read from a descriptor in memory (not a kernel argument) is a
+// generic one, and accesses through it compile to FLAT instructions, which
+// count against the LDS wait counter too: every LDS wait of a line solve
+// then also waits for them.  gref(p) is *p addressed as global memory (p
+// must point to device memory), so the access is a global load or store.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T& gref(T* p) {
+  return *(__attribute__((address_space(1))) T*)p;
+}
+
+
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 
 #define EMX_TRY_HIP(call)                                                   \

@@ -287,15 +287,15 @@ __device__ __forceinline__ double bdiff(const double* phi, PDims g, int ax,
   const long long sxy = (long long)g.x * g.y;
   const long long id = (long long)x + (long long)g.x * y + sxy * z;
   if (ax == 0) {
-    if (x == 0) return ih * (phi[id + 1] - phi[id]);
-    return ih * (phi[id] - phi[id - 1]);
+    if (x == 0) return ih * (gref(phi + id + 1) - gref(phi + id));
+    return ih * (gref(phi + id) - gref(phi + id - 1));
   }
   if (ax == 1) {
-    if (y == 0) return ih * (phi[id + g.x] - phi[id]);
-    return ih * (phi[id] - phi[id - g.x]);
+    if (y == 0) return ih * (gref(phi + id + g.x) - gref(phi + id));
+    return ih * (gref(phi + id) - gref(phi + id - g.x));
   }
-  if (z == 0) return ih * (phi[id + sxy] - phi[id]);
-  return ih * (phi[id] - phi[id - sxy]);
+  if (z == 0) return ih * (gref(phi + id + sxy) - gref(phi + id));
+  return ih * (gref(phi + id) - gref(phi + id - sxy));
 }
 
 // The backward differences (Dev1thO_Downwind, lsTools.cu:53-95) times the
@@ -314,7 +314,7 @@ __device__ __forceinline__ void axis_bd(const double* src, unsigned stride, int 
   for (int o = 0; o < 7; o++) {
     int t = q - 3 + o;
     t = t < 0 ? 0 : (t > n - 1 ? n - 1 : t);
-    s[o] = src[uid + (unsigned)(t - q) * stride];
+    s[o] = gref(src + (uid + (unsigned)(t - q) * stride));
   }
 #pragma unroll
   for (int o = 0; o < 6; o++) bd[o] = ihc * (s[o + 1] - s[o]) * m;
@@ -335,7 +335,7 @@ __device__ __forceinline__ void axis_bd_inner(const double* src, unsigned stride
   const int st = (int)stride;
   double s[7];
 #pragma unroll
-  for (int o = 0; o < 7; o++) s[o] = c0[(o - 3) * st];
+  for (int o = 0; o < 7; o++) s[o] = gref(c0 + (o - 3) * st);
 #pragma unroll
   for (int o = 0; o < 6; o++) bd[o] = ihc * (s[o + 1] - s[o]) * m;
 }
@@ -414,7 +414,8 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
   const unsigned vid = (unsigned)mirror(x, (int)nx) +
                        nx * ((unsigned)mirror(y, (int)ny) + ny * (unsigned)mirror(z, (int)nz));
   const unsigned vN = nx * ny * nz;
-  const double met[3] = {jbn[vid], jbn[(size_t)4 * vN + vid], jbn[(size_t)8 * vN + vid]};
+  const double met[3] = {gref(jbn + vid), gref(jbn + (size_t)4 * vN + vid),
+                         gref(jbn + (size_t)8 * vN + vid)};
   double r[3];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
@@ -422,7 +423,7 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
     // derivative where the velocity is positive and the right-biased one
     // where it is negative, so only that one is formed:
     //   0.5(s+1) v L + 0.5|s-1| v R  =  v L (v > 0),  v R (v < 0),  0
-    const double v = vel[(size_t)c * vN + vid];
+    const double v = gref(vel + (size_t)c * vN + vid);
     if (v == 0.0) {
       r[c] = 0.0;
       continue;
@@ -444,15 +445,15 @@ __device__ __forceinline__ void weno_stage_point(int stage, const double* src,
     r[c] = v * weno5(b[0], b[1], b[2], b[3], b[4]);
   }
   const double rs = r[0] + r[1] + r[2];
-  const double sv = src[id];
+  const double sv = gref(src + id);
   double o;
   if (stage == 0) o = sv - dt * rs;
-  else if (stage == 1) o = 0.75 * P[id] + 0.25 * (sv - dt * rs);
-  else o = (P[id] + 2.0 * (sv - dt * rs)) * (1.0 / 3.0);
+  else if (stage == 1) o = 0.75 * gref(P + id) + 0.25 * (sv - dt * rs);
+  else o = (gref(P + id) + 2.0 * (sv - dt * rs)) * (1.0 / 3.0);
   if (phi_int)
-    phi_int[(unsigned)(x - GC) + nx * ((unsigned)(y - GC) + ny * (unsigned)(z - GC))] = o;
+    gref(phi_int + ((unsigned)(x - GC) + nx * ((unsigned)(y - GC) + ny * (unsigned)(z - GC)))) = o;
   else
-    out[id] = o;
+    gref(out + id) = o;
 }
 
 __global__ void k_weno_stage(int stage, const double* src, const double* P,
@@ -586,7 +587,7 @@ k_batch_pad(const LsBatchGrid* grids, const int* blk_start, int ngrid) {
             k = (int)(row / (unsigned)g.y);
   const long long src = mirror(i, G.nx) + (long long)G.nx * (mirror(j, G.ny) +
                                                              (long long)G.ny * mirror(k, G.nz));
-  G.P[p] = G.phi[src];
+  gref(G.P + p) = gref(G.phi + src);
 }
 
 __global__ void __launch_bounds__(256)

@@ -56,17 +56,19 @@ struct ProbGradDiv {
   double* div;
   long long N;
   int dir;
+  // Every access goes through gref: the batched launches read this problem
+  // from a descriptor array, and its pointers would otherwise be generic.
   __device__ __forceinline__ void load(long long p, double* v) const {
 #pragma unroll
-    for (int c = 0; c < 3; c++) v[c] = u[c * N + p];
-    v[3] = jbn[9 * N + p] * (jbn[(4 * dir) * N + p] * v[dir]);
+    for (int c = 0; c < 3; c++) v[c] = gref(u + c * N + p);
+    v[3] = gref(jbn + 9 * N + p) * (gref(jbn + (4 * dir) * N + p) * v[dir]);
   }
   __device__ __forceinline__ void store(long long p, const double* v) const {
 #pragma unroll
-    for (int c = 0; c < 3; c++) dvel[(3 * dir + c) * N + p] = v[c];
-    if (dir == 0) acc[p] = v[3];
-    else if (dir == 1) acc[p] = acc[p] + v[3];
-    else div[p] = jbn[10 * N + p] * (acc[p] + v[3]);
+    for (int c = 0; c < 3; c++) gref(dvel + (3 * dir + c) * N + p) = v[c];
+    if (dir == 0) gref(acc + p) = v[3];
+    else if (dir == 1) gref(acc + p) = gref(acc + p) + v[3];
+    else gref(div + p) = gref(jbn + 10 * N + p) * (gref(acc + p) + v[3]);
   }
   // Q points p + i*sl at once: the accumulator (and 1/J) reads first, as
   // ProbAccum::store_batch.  Same arithmetic as store().
@@ -78,18 +80,18 @@ struct ProbGradDiv {
     if (dir > 0) {
 #pragma unroll
       for (int i = 0; i < Q; i++) {
-        a[i] = acc[p + i * sl];
-        q[i] = dir == 2 ? jbn[10 * N + p + i * sl] : 0.0;
+        a[i] = gref(acc + p + i * sl);
+        q[i] = dir == 2 ? gref(jbn + 10 * N + p + i * sl) : 0.0;
       }
     }
 #pragma unroll
     for (int i = 0; i < Q; i++) {
       const long long pi = p + i * sl;
 #pragma unroll
-      for (int c = 0; c < 3; c++) dvel[(3 * dir + c) * N + pi] = v[i][c];
-      if (dir == 0) acc[pi] = v[i][3];
-      else if (dir == 1) acc[pi] = a[i] + v[i][3];
-      else div[pi] = q[i] * (a[i] + v[i][3]);
+      for (int c = 0; c < 3; c++) gref(dvel + (3 * dir + c) * N + pi) = v[i][c];
+      if (dir == 0) gref(acc + pi) = v[i][3];
+      else if (dir == 1) gref(acc + pi) = a[i] + v[i][3];
+      else gref(div + pi) = q[i] * (a[i] + v[i][3]);
     }
   }
 };
