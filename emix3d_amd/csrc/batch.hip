@@ -194,8 +194,9 @@ int plan_wave_lines(emx_batch* b, BatchShape s) {
     const Dims& d = b->dims[i];
     const DirGeom g = dir_geom(d, 0);
     BatchWaveLines<ProbGradDiv>& L = H[i];
+    // the shortest chunk of 1, 2, 3, 4, 8 (at most s.C) whose 64 lanes hold the line
     int C = 1;
-    while (C < s.C && g.n > 64 * C) C *= 2;
+    while (C < s.C && g.n > 64 * C) C = C == 2 ? 3 : (C == 3 ? 4 : 2 * C);
     EMX_TRY(get_line_tab(b->scheme, g.n, C, KIND_WAVE, &L.T));
     L.nlines = g.nlines;
     L.nyl = g.nyl;

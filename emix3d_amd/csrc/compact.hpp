@@ -619,7 +619,9 @@ k_lines_batched(const BatchLines<Prob>* grids, const int* blk_start, int ngrid) 
 // memory): workgroup b belongs to grid g as above and is workgroup
 // b - blk_start[g] of the nblk that grid's lines are dealt to (its four
 // waves take every (4*nblk)-th line).  Each grid has its own chunk length
-// (its table's C, at most CMAX): the branch is uniform over a workgroup.
+// (its table's C, at most CMAX; 3 as well as the powers of two, so that
+// lines of 129-192 points fill three quarters of a wave's 256 slots at
+// least): the branch is uniform over a workgroup.
 template <class Prob>
 struct BatchWaveLines {
   Prob prob;
@@ -647,6 +649,9 @@ k_lines_batched_wave(const BatchWaveLines<Prob>* grids, const int* blk_start, in
                                              G.ih, blk, G.nblk);
   else if (CMAX >= 4 && G.T.C == 4)
     lines_wave_body<S, (CMAX >= 4 ? 4 : 1)>(prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
+                                             G.ih, blk, G.nblk);
+  else if (CMAX >= 4 && G.T.C == 3)
+    lines_wave_body<S, (CMAX >= 4 ? 3 : 1)>(prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
                                              G.ih, blk, G.nblk);
   else if (CMAX >= 2 && G.T.C == 2)
     lines_wave_body<S, (CMAX >= 2 ? 2 : 1)>(prob, G.T, G.nlines, G.nyl, G.sj, G.sk, G.sl,
