@@ -8,7 +8,7 @@ per phase over all boxes, replayed from a HIP graph.
   * graph replays, with eager work and synchronisations between them, equal
     the eager result each time;
   * the wavefront kind along x with each box on its own chunk length (1, 2,
-    4 and 8 points per lane in one launch), and its shape checks;
+    3, 4 and 8 points per lane in one launch), and its shape checks;
   * metrics changed in place after the bind are used by the next stage;
   * the autotuner keeps a built shape and the results do not change;
   * the single-grid entry emx_grad_div gives the same fields.
@@ -116,9 +116,9 @@ def test_batch_stage_other_schemes(oracle, scheme, sid):
 
 
 def test_batch_wave_chunk_per_box(oracle):
-    # x lines of 20, 70, 130 and 260 points: 1, 2, 4 and 8 points per lane in
-    # one launch of the wavefront kind (each box its shortest chunk)
-    dims = [(70, 9, 8), (260, 6, 5), (20, 8, 7), (130, 7, 6)]
+    # x lines of 20, 70, 130, 200 and 260 points: 1, 2, 3, 4 and 8 points per
+    # lane in one launch of the wavefront kind (each box its shortest chunk)
+    dims = [(70, 9, 8), (260, 6, 5), (20, 8, 7), (130, 7, 6), (200, 6, 5)]
     b, boxes = make_batch(dims)
     refs = [reference(oracle, z) for z in boxes]
     assert b.get_plan(0) == (0, 8)  # the only wavefront shape that fits 260
